@@ -1,0 +1,318 @@
+// afft_device.hpp -- device-side building blocks of the gfx950 FFT kernels.
+//
+// Complex values are HIP float2/double2 (re, im): the interleaved Vec2 layout of
+// Complex Float / Complex Double (src/Data/Array/Accelerate/Math/FFT/Type.hs:32).
+//
+// Every kernel computes the *forward* DFT X[k] = sum_n x[n] exp(-2 pi i nk/N).  The opposite
+// direction (Reverse / Inverse, FFT/Mode.hs:21-26) is conj(F(conj x)): the conjugations are
+// folded into the load and the store, so one set of butterflies serves both directions.
+//
+// Stockham autosort formulation (Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC'08): pass p of radix R with Ns = product of the earlier
+// radices reads butterfly b's inputs at b + r*N/R, multiplies input r by W_{Ns R}^{r (b mod Ns)},
+// applies DFT_R and writes output q to (b - b mod Ns)*R + (b mod Ns) + q*Ns.
+//
+// Register layout.  A transform of length N is owned by T = N/E threads, each holding E
+// elements.  Thread t holds element t + i*T in reg[i] ("natural" layout).  For a pass of radix R
+// (R divides E), butterfly j of thread t is b = t + j*T and uses reg[j + r*E/R]; the last pass
+// writes exactly the natural layout, so the first pass loads straight from global memory and the
+// last pass stores straight to it, and only the passes in between exchange through LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afft {
+
+template <typename Real> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+
+template <typename V> __device__ __forceinline__ V c_make(decltype(V::x) re, decltype(V::x) im)
+{
+    V v;
+    v.x = re;
+    v.y = im;
+    return v;
+}
+template <typename V> __device__ __forceinline__ V c_add(V a, V b) { return c_make<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V c_sub(V a, V b) { return c_make<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V c_mul(V a, V b)
+{
+    return c_make<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename V> __device__ __forceinline__ V c_mulc(V a, V b)
+{
+    return c_make<V>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename V> __device__ __forceinline__ V c_conj(V a) { return c_make<V>(a.x, -a.y); }
+// a * (-i)
+template <typename V> __device__ __forceinline__ V c_mul_negi(V a) { return c_make<V>(a.y, -a.x); }
+template <typename V> __device__ __forceinline__ V c_scale(V a, decltype(V::x) s) { return c_make<V>(a.x * s, a.y * s); }
+
+// ----------------------------------------------------------------------------------------------
+// Forward DFT_R butterflies on v[0..R-1], in place.
+
+template <int R> struct dft;
+
+template <> struct dft<1> {
+    template <typename V> __device__ __forceinline__ static void run(V *) {}
+};
+
+template <> struct dft<2> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        V a = v[0], b = v[1];
+        v[0] = c_add(a, b);
+        v[1] = c_sub(a, b);
+    }
+};
+
+template <> struct dft<3> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        using Real = decltype(V::x);
+        const Real h = (Real)0.86602540378443864676372317075293618; // sqrt(3)/2
+        V t1 = c_add(v[1], v[2]);
+        V d = c_sub(v[1], v[2]);
+        V t2 = c_make<V>(v[0].x - (Real)0.5 * t1.x, v[0].y - (Real)0.5 * t1.y);
+        V m = c_make<V>(h * d.y, -h * d.x); // -i * (sqrt3/2) * d
+        v[0] = c_add(v[0], t1);
+        v[1] = c_add(t2, m);
+        v[2] = c_sub(t2, m);
+    }
+};
+
+template <> struct dft<4> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        V s0 = c_add(v[0], v[2]), d0 = c_sub(v[0], v[2]);
+        V s1 = c_add(v[1], v[3]), d1 = c_mul_negi(c_sub(v[1], v[3]));
+        v[0] = c_add(s0, s1);
+        v[2] = c_sub(s0, s1);
+        v[1] = c_add(d0, d1);
+        v[3] = c_sub(d0, d1);
+    }
+};
+
+template <> struct dft<5> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        using Real = decltype(V::x);
+        const Real c1 = (Real)0.30901699437494742410229341718281906;  // cos(2pi/5)
+        const Real c2 = (Real)-0.80901699437494742410229341718281906; // cos(4pi/5)
+        const Real s1 = (Real)0.95105651629515357211643933337938214;  // sin(2pi/5)
+        const Real s2 = (Real)0.58778525229247312916870595463907277;  // sin(4pi/5)
+        V t1 = c_add(v[1], v[4]), d1 = c_sub(v[1], v[4]);
+        V t2 = c_add(v[2], v[3]), d2 = c_sub(v[2], v[3]);
+        V a1 = c_make<V>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        V a2 = c_make<V>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        V b1 = c_make<V>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        V b2 = c_make<V>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+        v[0] = c_make<V>(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+        // y1 = a1 - i b1, y4 = a1 + i b1, y2 = a2 - i b2, y3 = a2 + i b2
+        v[1] = c_make<V>(a1.x + b1.y, a1.y - b1.x);
+        v[4] = c_make<V>(a1.x - b1.y, a1.y + b1.x);
+        v[2] = c_make<V>(a2.x + b2.y, a2.y - b2.x);
+        v[3] = c_make<V>(a2.x - b2.y, a2.y + b2.x);
+    }
+};
+
+template <> struct dft<8> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        using Real = decltype(V::x);
+        const Real r = (Real)0.70710678118654752440084436210484904; // 1/sqrt(2)
+        V e[4] = {v[0], v[2], v[4], v[6]};
+        V o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4>::run(e);
+        dft<4>::run(o);
+        // o[k] *= W8^k: W8 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = (-1 - i)/sqrt2
+        V o1 = c_make<V>(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+        V o2 = c_mul_negi(o[2]);
+        V o3 = c_make<V>(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+        v[0] = c_add(e[0], o[0]);
+        v[4] = c_sub(e[0], o[0]);
+        v[1] = c_add(e[1], o1);
+        v[5] = c_sub(e[1], o1);
+        v[2] = c_add(e[2], o2);
+        v[6] = c_sub(e[2], o2);
+        v[3] = c_add(e[3], o3);
+        v[7] = c_sub(e[3], o3);
+    }
+};
+
+template <> struct dft<16> {
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        using Real = decltype(V::x);
+        const Real c8 = (Real)0.92387953251128675612818318939678829;  // cos(pi/8)
+        const Real s8 = (Real)0.38268343236508977172845998403039887;  // sin(pi/8)
+        const Real r2 = (Real)0.70710678118654752440084436210484904;  // 1/sqrt(2)
+        // n = r1 + 4 n2, k = k1 + 4 k2
+        V a[4][4];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; ++r1) {
+            V t[4] = {v[r1], v[r1 + 4], v[r1 + 8], v[r1 + 12]};
+            dft<4>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) a[r1][k1] = t[k1];
+        }
+        // twiddles W16^(r1*k1)
+        const V w1 = c_make<V>(c8, -s8), w2 = c_make<V>(r2, -r2), w3 = c_make<V>(s8, -c8);
+        const V w6 = c_make<V>(-r2, -r2), w9 = c_make<V>(-c8, s8);
+        a[1][1] = c_mul(a[1][1], w1);
+        a[1][2] = c_mul(a[1][2], w2);
+        a[1][3] = c_mul(a[1][3], w3);
+        a[2][1] = c_mul(a[2][1], w2);
+        a[2][2] = c_mul_negi(a[2][2]);
+        a[2][3] = c_mul(a[2][3], w6);
+        a[3][1] = c_mul(a[3][1], w3);
+        a[3][2] = c_mul(a[3][2], w6);
+        a[3][3] = c_mul(a[3][3], w9);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            V t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            dft<4>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
+// ----------------------------------------------------------------------------------------------
+// Pass descriptors (shared by host and device).
+
+constexpr int kMaxPasses = 24;
+
+// Largest workgroup a kernel holding E complex elements per thread is compiled for.  The launch
+// bound sets the VGPR budget (512 per SIMD lane slot / waves per SIMD): 1024 threads -> 128
+// VGPRs, 512 -> 256, 256 -> 512.  One FP32 complex is 2 VGPRs, one FP64 complex 4; the
+// butterflies need about twice the data registers.
+__host__ __device__ constexpr int max_block_threads(bool f64, int E)
+{
+    return (f64 ? 2 * E : E) <= 16 ? 1024 : ((f64 ? 2 * E : E) <= 32 ? 512 : 256);
+}
+
+struct PassTable {
+    int n;                       // transform length
+    int npass;                   // number of Stockham passes
+    int radix[kMaxPasses];       // radix of each pass
+    int tw_off[kMaxPasses];      // offset (in elements) of the pass's twiddles, stride R per k
+};
+
+// LDS address of element `pos` of the transform whose LDS base is `base`: one padding element
+// after every 16 spreads the strided Stockham writes over the banks.
+__device__ __forceinline__ int lds_pad(int pos) { return pos + (pos >> 4); }
+__host__ __device__ constexpr int lds_padded_len(int n) { return n + (n >> 4) + 1; }
+
+// Twiddle-multiply + DFT_R for every butterfly the thread owns in the current pass.
+template <int E, int R, typename V>
+__device__ __forceinline__ void pass_butterflies(V (&reg)[E], const V *__restrict__ tw, int Ns,
+                                                 int t, int T)
+{
+    constexpr int B = E / R;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        V v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = reg[j + r * B];
+        if (Ns > 1) {
+            const int b = t + j * T;
+            const int k = b % Ns;
+            const V *w = tw + (int64_t)k * R;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], w[r]);
+        }
+        dft<R>::run(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
+    }
+}
+
+// Scatter the pass outputs to their Stockham positions in LDS.
+// lds(pos) = lds_base[lds_pad(pos) * es]  (es = 1 for a private row, C for interleaved columns)
+template <int E, int R, typename V>
+__device__ __forceinline__ void pass_scatter(const V (&reg)[E], V *lds_base, int es, int Ns, int t,
+                                             int T)
+{
+    constexpr int B = E / R;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const int b = t + j * T;
+        const int k = b % Ns;
+        const int dst = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds_base[lds_pad(dst + r * Ns) * es] = reg[j + r * B];
+    }
+}
+
+template <int E, typename V>
+__device__ __forceinline__ void natural_read(V (&reg)[E], const V *lds_base, int es, int t, int T)
+{
+#pragma unroll
+    for (int i = 0; i < E; ++i) reg[i] = lds_base[lds_pad(t + i * T) * es];
+}
+
+template <int E, typename V>
+__device__ __forceinline__ void natural_write(const V (&reg)[E], V *lds_base, int es, int t, int T)
+{
+#pragma unroll
+    for (int i = 0; i < E; ++i) lds_base[lds_pad(t + i * T) * es] = reg[i];
+}
+
+// One pass of radix R, dispatched at run time.  Only radices that divide E are instantiated.
+template <int E, typename V>
+__device__ __forceinline__ void pass_dispatch(int R, V (&reg)[E], const V *__restrict__ tw, int Ns,
+                                              int t, int T)
+{
+    switch (R) {
+    case 2: if constexpr (E % 2 == 0) pass_butterflies<E, 2>(reg, tw, Ns, t, T); break;
+    case 3: if constexpr (E % 3 == 0) pass_butterflies<E, 3>(reg, tw, Ns, t, T); break;
+    case 4: if constexpr (E % 4 == 0) pass_butterflies<E, 4>(reg, tw, Ns, t, T); break;
+    case 5: if constexpr (E % 5 == 0) pass_butterflies<E, 5>(reg, tw, Ns, t, T); break;
+    case 8: if constexpr (E % 8 == 0) pass_butterflies<E, 8>(reg, tw, Ns, t, T); break;
+    case 16: if constexpr (E % 16 == 0) pass_butterflies<E, 16>(reg, tw, Ns, t, T); break;
+    default: break;
+    }
+}
+
+template <int E, typename V>
+__device__ __forceinline__ void scatter_dispatch(int R, const V (&reg)[E], V *lds_base, int es,
+                                                 int Ns, int t, int T)
+{
+    switch (R) {
+    case 2: if constexpr (E % 2 == 0) pass_scatter<E, 2>(reg, lds_base, es, Ns, t, T); break;
+    case 3: if constexpr (E % 3 == 0) pass_scatter<E, 3>(reg, lds_base, es, Ns, t, T); break;
+    case 4: if constexpr (E % 4 == 0) pass_scatter<E, 4>(reg, lds_base, es, Ns, t, T); break;
+    case 5: if constexpr (E % 5 == 0) pass_scatter<E, 5>(reg, lds_base, es, Ns, t, T); break;
+    case 8: if constexpr (E % 8 == 0) pass_scatter<E, 8>(reg, lds_base, es, Ns, t, T); break;
+    case 16: if constexpr (E % 16 == 0) pass_scatter<E, 16>(reg, lds_base, es, Ns, t, T); break;
+    default: break;
+    }
+}
+
+// The whole forward transform of the rows held in registers (natural layout in, natural layout
+// out).  All threads of the workgroup must call it (it contains barriers); threads that own no
+// transform (tail of the batch) still take part.  `lds_base`/`es` address this thread's
+// transform in LDS; `tw` holds the pass twiddles (pass p at tw + pt.tw_off[p]).
+template <int E, typename V>
+__device__ __forceinline__ void fft_core(V (&reg)[E], V *lds_base, int es, const PassTable &pt,
+                                         const V *__restrict__ tw, int t, int T)
+{
+    int Ns = 1;
+    for (int p = 0; p < pt.npass; ++p) {
+        const int R = pt.radix[p];
+        pass_dispatch<E>(R, reg, tw + pt.tw_off[p], Ns, t, T);
+        if (p + 1 < pt.npass) {
+            __syncthreads(); // everyone is done reading the previous exchange
+            scatter_dispatch<E>(R, reg, lds_base, es, Ns, t, T);
+            __syncthreads();
+            natural_read<E>(reg, lds_base, es, t, T);
+        }
+        Ns *= R;
+    }
+}
+
+} // namespace afft

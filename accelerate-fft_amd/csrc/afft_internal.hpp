@@ -1,0 +1,89 @@
+// afft_internal.hpp -- launch descriptors shared by the planner (afft_plan.cpp) and the kernels
+// (afft_kernels.hip).  Host-visible, no device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "afft_device.hpp"
+
+namespace afft {
+
+// Addressing of a batch of 1D transforms.  Transform j decomposes as
+//   j0 = j % cnt0,  j1 = (j / cnt0) % cnt1,  j2 = j / (cnt0 * cnt1)
+// and its element i lives at  j0*st0 + j1*st1 + j2*st2 + i*es  (in elements).
+struct Addr {
+    int64_t cnt0, cnt1;
+    int64_t st0, st1, st2;
+    int64_t es;
+};
+
+// One batched Stockham launch: ntrans transforms of length pt.n, C of them per workgroup.
+struct FftArgs {
+    const void *in;
+    void *out;
+    const void *tw;      // pass twiddles (element type), pass p at tw + pt.tw_off[p]
+    int64_t ntrans;
+    Addr ain, aout;
+    PassTable pt;
+    int E;               // elements per thread (T = n / E threads per transform)
+    int C;               // transforms per workgroup
+    int lane_col;        // 1: the transform index varies fastest across lanes (strided axes)
+    int lds_interleave;  // 1: LDS element (c, pos) at pad(pos)*C + c; 0: c*padded(n) + pad(pos)
+    int conj_in;         // conjugate on load (direction +1)
+    int conj_out;        // conjugate on store (direction +1)
+    double scale;        // applied on store
+    // four-step twiddle on store: out[k] *= W_L^{(jl * k) mod L}, jl = level-`post_level` index
+    // of the transform (0 -> j0, 1 -> j1); W from the two-level table hi[e >> s] * lo[e & (2^s-1)]
+    int64_t post_L;      // 0 = none
+    int post_level;
+    int post_shift;
+    const void *post_hi;
+    const void *post_lo;
+};
+
+// Fused single-workgroup Bluestein (chirp-z) launch for a length n that is not 5-smooth:
+// pt describes the padded length M (5-smooth, M >= 2n - 1).
+struct BluesteinArgs {
+    const void *in;
+    void *out;
+    const void *tw;      // pass twiddles for length M
+    const void *chirp;   // c[m] = exp(-i pi m^2 / n), m < n
+    const void *filt;    // FFT_M(b) / M, b[m] = conj(c[|m|]) wrapped
+    int64_t ntrans;
+    int64_t n;
+    Addr ain, aout;
+    PassTable pt;
+    int E, C;
+    int lane_col, lds_interleave;
+    int conj_in, conj_out;
+    double scale;
+};
+
+// Launchers (afft_kernels.hip).  They return a hipError_t.
+hipError_t launch_stockham(int dtype, const FftArgs &a, int block, int64_t grid, size_t lds,
+                           hipStream_t s);
+hipError_t launch_bluestein(int dtype, const BluesteinArgs &a, int block, int64_t grid, size_t lds,
+                            hipStream_t s);
+// FP64 twiddle generation, rounded to the element type.
+hipError_t gen_pass_twiddles(int dtype, void *out, const PassTable &pt, hipStream_t s);
+// out[i] = W_L^{i * step} for i < count (two-level four-step tables).
+hipError_t gen_power_table(int dtype, void *out, int64_t L, int64_t step, int64_t count,
+                           hipStream_t s);
+// out[m] = exp(-i pi (m^2 mod 2n) / n) for m < n (chirp), in the element type.
+hipError_t gen_chirp(int dtype, void *out, int64_t n, hipStream_t s);
+// out (complex double, length M) = b[m]: conj(chirp[m]) for m < n, conj(chirp[M-m]) for
+// M - m < n, 0 otherwise, computed in FP64.
+hipError_t gen_bluestein_kernel_f64(void *out, int64_t n, int64_t M, hipStream_t s);
+// out (element type) = in (complex double) * scale, count elements.
+hipError_t convert_scale_from_f64(int dtype, void *out, const void *in, int64_t count, double scale,
+                                  hipStream_t s);
+// y = x * h (elementwise, complex), count elements; h indexed modulo hcount.
+hipError_t pointwise_mul(int dtype, void *y, const void *x, const void *h, int64_t count,
+                         int64_t hcount, int conj_h, double scale, hipStream_t s);
+
+// The elements-per-thread values that have kernel instantiations.
+extern const int kElemsPerThread[];
+extern const int kNumElemsPerThread;
+
+} // namespace afft

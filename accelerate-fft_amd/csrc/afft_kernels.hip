@@ -1,0 +1,410 @@
+// afft_kernels.hip -- gfx950 kernels of the FFT engine.
+//
+//   afft_stockham_<f32|f64>_e<E>   batched single-workgroup LDS Stockham transform (rows,
+//                                  strided axes, and both passes of the four-step scheme)
+//   afft_bluestein_<f32|f64>_e<E>  fused chirp-z transform for lengths that are not 5-smooth
+//   afft_gen_*                     FP64 twiddle / chirp generation, rounded to the element type
+//
+// The path is HBM-bound (about 0.3 log2 N flop per byte), so nothing here uses MFMA: the design
+// goal is one HBM read and one HBM write per element per launch, with the passes of a transform
+// exchanged through LDS.
+#include <hip/hip_runtime.h>
+
+#include "afft_device.hpp"
+#include "afft_internal.hpp"
+
+namespace afft {
+
+template <int E> struct ElemsTag {};
+
+__device__ __forceinline__ int64_t addr_base(const Addr &a, int64_t j)
+{
+    const int64_t j0 = j % a.cnt0;
+    const int64_t q = j / a.cnt0;
+    const int64_t j1 = q % a.cnt1;
+    const int64_t j2 = q / a.cnt1;
+    return j0 * a.st0 + j1 * a.st1 + j2 * a.st2;
+}
+
+__device__ __forceinline__ int64_t addr_level(const Addr &a, int64_t j, int level)
+{
+    return level == 0 ? j % a.cnt0 : (j / a.cnt0) % a.cnt1;
+}
+
+// thread -> (transform slot c in the workgroup, thread t within the transform)
+__device__ __forceinline__ void split_tid(int tid, int C, int T, int lane_col, int &c, int &t)
+{
+    if (lane_col) {
+        c = tid % C;
+        t = tid / C;
+    } else {
+        t = tid % T;
+        c = tid / T;
+    }
+}
+
+template <typename V>
+__device__ __forceinline__ V *lds_slot(V *lds, int c, int C, int n, int interleave, int &es)
+{
+    if (interleave) {
+        es = C;
+        return lds + c;
+    }
+    es = 1;
+    return lds + (int64_t)c * lds_padded_len(n);
+}
+
+template <typename V>
+__device__ __forceinline__ V power_lookup(const V *hi, const V *lo, int64_t e, int shift)
+{
+    const V a = hi[e >> shift];
+    const V b = lo[e & ((int64_t(1) << shift) - 1)];
+    return c_mul(a, b);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Batched Stockham transform.
+
+template <typename Real, int E>
+__global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_stockham(const FftArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    extern __shared__ __align__(16) unsigned char afft_smem[];
+    V *lds = reinterpret_cast<V *>(afft_smem);
+
+    const int n = a.pt.n;
+    const int T = n / E;
+    int c, t;
+    split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
+    const int64_t j = (int64_t)blockIdx.x * a.C + c;
+    const bool active = (c < a.C) && (j < a.ntrans);
+
+    V reg[E];
+    if (active) {
+        const V *in = reinterpret_cast<const V *>(a.in);
+        const int64_t base = addr_base(a.ain, j);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            V v = in[base + (int64_t)(t + i * T) * a.ain.es];
+            reg[i] = a.conj_in ? c_conj(v) : v;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) reg[i] = c_make<V>(0, 0);
+    }
+
+    int es;
+    V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, n, a.lds_interleave, es);
+    fft_core<E>(reg, slot, es, a.pt, reinterpret_cast<const V *>(a.tw), t, T);
+
+    if (active) {
+        V *out = reinterpret_cast<V *>(a.out);
+        const int64_t base = addr_base(a.aout, j);
+        const Real sc = (Real)a.scale;
+        const int64_t jl = a.post_L ? addr_level(a.ain, j, a.post_level) : 0;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int k = t + i * T;
+            V v = reg[i];
+            if (a.post_L) {
+                const int64_t e = (jl * (int64_t)k) % a.post_L;
+                v = c_mul(v, power_lookup(reinterpret_cast<const V *>(a.post_hi),
+                                          reinterpret_cast<const V *>(a.post_lo), e, a.post_shift));
+            }
+            if (a.conj_out) v = c_conj(v);
+            out[base + (int64_t)k * a.aout.es] = c_scale(v, sc);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Fused Bluestein: X[k] = c[k] * IFFT_M( FFT_M(x c) * FFT_M(b)/M )[k],  c[m] = exp(-i pi m^2/n).
+// The inverse FFT is conj(FFT(conj(.))) so both transforms use the forward passes.
+
+template <typename Real, int E>
+__global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_bluestein(const BluesteinArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    extern __shared__ __align__(16) unsigned char afft_smem[];
+    V *lds = reinterpret_cast<V *>(afft_smem);
+
+    const int M = a.pt.n;
+    const int T = M / E;
+    int c, t;
+    split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
+    const int64_t j = (int64_t)blockIdx.x * a.C + c;
+    const bool active = (c < a.C) && (j < a.ntrans);
+    const V *chirp = reinterpret_cast<const V *>(a.chirp);
+    const V *filt = reinterpret_cast<const V *>(a.filt);
+    const V *tw = reinterpret_cast<const V *>(a.tw);
+
+    V reg[E];
+    if (active) {
+        const V *in = reinterpret_cast<const V *>(a.in);
+        const int64_t base = addr_base(a.ain, j);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int m = t + i * T;
+            V v = c_make<V>(0, 0);
+            if (m < a.n) {
+                v = in[base + (int64_t)m * a.ain.es];
+                if (a.conj_in) v = c_conj(v);
+                v = c_mul(v, chirp[m]);
+            }
+            reg[i] = v;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) reg[i] = c_make<V>(0, 0);
+    }
+
+    int es;
+    V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, M, a.lds_interleave, es);
+    fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+#pragma unroll
+    for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
+    fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+
+    if (active) {
+        V *out = reinterpret_cast<V *>(a.out);
+        const int64_t base = addr_base(a.aout, j);
+        const Real sc = (Real)a.scale;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int m = t + i * T;
+            if (m < a.n) {
+                V v = c_mul(c_conj(reg[i]), chirp[m]);
+                if (a.conj_out) v = c_conj(v);
+                out[base + (int64_t)m * a.aout.es] = c_scale(v, sc);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Twiddle / chirp generation in FP64.
+
+template <typename Real>
+__global__ void afft_gen_pass_twiddles(typename cplx_of<Real>::type *out, const PassTable pt)
+{
+    using V = typename cplx_of<Real>::type;
+    int Ns = 1;
+    for (int p = 0; p < pt.npass; ++p) {
+        const int R = pt.radix[p];
+        const int L = Ns * R;
+        for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < L; idx += gridDim.x * blockDim.x) {
+            const int k = idx / R, r = idx % R;
+            const int e = r * k; // < L, exact
+            double s, co;
+            sincospi(-2.0 * (double)e / (double)L, &s, &co);
+            out[pt.tw_off[p] + idx] = c_make<V>((Real)co, (Real)s);
+        }
+        Ns = L;
+    }
+}
+
+template <typename Real>
+__global__ void afft_gen_power_table(typename cplx_of<Real>::type *out, int64_t L, int64_t step,
+                                     int64_t count)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        // exponent reduced exactly in integers; i*step < 2^62 for every table built here
+        const int64_t e = (i * step) % L;
+        double s, co;
+        sincospi(-2.0 * (double)e / (double)L, &s, &co);
+        out[i] = c_make<V>((Real)co, (Real)s);
+    }
+}
+
+template <typename Real>
+__global__ void afft_gen_chirp(typename cplx_of<Real>::type *out, int64_t n)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n;
+         m += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = (m * m) % (2 * n); // exp(-i pi m^2/n) has period 2n in m^2
+        double s, co;
+        sincospi(-(double)e / (double)n, &s, &co);
+        out[m] = c_make<V>((Real)co, (Real)s);
+    }
+}
+
+__global__ void afft_gen_bluestein_kernel(double2 *out, int64_t n, int64_t M)
+{
+    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < M;
+         m += (int64_t)gridDim.x * blockDim.x) {
+        int64_t src = -1;
+        if (m < n) src = m;
+        else if (M - m < n) src = M - m;
+        double2 v = make_double2(0.0, 0.0);
+        if (src >= 0) {
+            const int64_t e = (src * src) % (2 * n);
+            double s, co;
+            sincospi(-(double)e / (double)n, &s, &co);
+            v = make_double2(co, -s); // conj(chirp)
+        }
+        out[m] = v;
+    }
+}
+
+template <typename Real>
+__global__ void afft_convert_scale(typename cplx_of<Real>::type *out, const double2 *in,
+                                   int64_t count, double scale)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = in[i];
+        out[i] = c_make<V>((Real)(v.x * scale), (Real)(v.y * scale));
+    }
+}
+
+template <typename Real>
+__global__ void afft_pointwise_mul(typename cplx_of<Real>::type *y, const typename cplx_of<Real>::type *x,
+                                   const typename cplx_of<Real>::type *h, int64_t count, int64_t hcount,
+                                   int conj_h, Real scale)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        V hv = h[i % hcount];
+        if (conj_h) hv = c_conj(hv);
+        y[i] = c_scale(c_mul(x[i], hv), scale);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Host launchers.
+
+const int kElemsPerThread[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32, 40, 48};
+const int kNumElemsPerThread = sizeof(kElemsPerThread) / sizeof(kElemsPerThread[0]);
+
+#define AFFT_FOR_EACH_E(X)                                                                       \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27)  \
+    X(30) X(32) X(40) X(48)
+
+template <typename Kern, typename Args>
+static hipError_t launch_one(Kern k, const Args &a, int block, int64_t grid, size_t lds, hipStream_t s)
+{
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t launch_stockham_t(const FftArgs &a, int block, int64_t grid, size_t lds, hipStream_t s)
+{
+    switch (a.E) {
+#define X(EV)                                                                                    \
+    case EV: return launch_one(afft_stockham<Real, EV>, a, block, grid, lds, s);
+        AFFT_FOR_EACH_E(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename Real>
+static hipError_t launch_bluestein_t(const BluesteinArgs &a, int block, int64_t grid, size_t lds,
+                                     hipStream_t s)
+{
+    switch (a.E) {
+#define X(EV)                                                                                    \
+    case EV: return launch_one(afft_bluestein<Real, EV>, a, block, grid, lds, s);
+        AFFT_FOR_EACH_E(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stockham(int dtype, const FftArgs &a, int block, int64_t grid, size_t lds, hipStream_t s)
+{
+    return dtype == 0 ? launch_stockham_t<float>(a, block, grid, lds, s)
+                      : launch_stockham_t<double>(a, block, grid, lds, s);
+}
+
+hipError_t launch_bluestein(int dtype, const BluesteinArgs &a, int block, int64_t grid, size_t lds,
+                            hipStream_t s)
+{
+    return dtype == 0 ? launch_bluestein_t<float>(a, block, grid, lds, s)
+                      : launch_bluestein_t<double>(a, block, grid, lds, s);
+}
+
+static int64_t gs_grid(int64_t count)
+{
+    int64_t g = (count + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return g;
+}
+
+hipError_t gen_pass_twiddles(int dtype, void *out, const PassTable &pt, hipStream_t s)
+{
+    const int64_t g = gs_grid(pt.n);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_gen_pass_twiddles<float>, dim3(g), dim3(256), 0, s, (float2 *)out, pt);
+    else
+        hipLaunchKernelGGL(afft_gen_pass_twiddles<double>, dim3(g), dim3(256), 0, s, (double2 *)out, pt);
+    return hipGetLastError();
+}
+
+hipError_t gen_power_table(int dtype, void *out, int64_t L, int64_t step, int64_t count, hipStream_t s)
+{
+    const int64_t g = gs_grid(count);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_gen_power_table<float>, dim3(g), dim3(256), 0, s, (float2 *)out, L, step, count);
+    else
+        hipLaunchKernelGGL(afft_gen_power_table<double>, dim3(g), dim3(256), 0, s, (double2 *)out, L, step, count);
+    return hipGetLastError();
+}
+
+hipError_t gen_chirp(int dtype, void *out, int64_t n, hipStream_t s)
+{
+    const int64_t g = gs_grid(n);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_gen_chirp<float>, dim3(g), dim3(256), 0, s, (float2 *)out, n);
+    else
+        hipLaunchKernelGGL(afft_gen_chirp<double>, dim3(g), dim3(256), 0, s, (double2 *)out, n);
+    return hipGetLastError();
+}
+
+hipError_t gen_bluestein_kernel_f64(void *out, int64_t n, int64_t M, hipStream_t s)
+{
+    hipLaunchKernelGGL(afft_gen_bluestein_kernel, dim3(gs_grid(M)), dim3(256), 0, s, (double2 *)out, n, M);
+    return hipGetLastError();
+}
+
+hipError_t convert_scale_from_f64(int dtype, void *out, const void *in, int64_t count, double scale,
+                                  hipStream_t s)
+{
+    const int64_t g = gs_grid(count);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_convert_scale<float>, dim3(g), dim3(256), 0, s, (float2 *)out,
+                           (const double2 *)in, count, scale);
+    else
+        hipLaunchKernelGGL(afft_convert_scale<double>, dim3(g), dim3(256), 0, s, (double2 *)out,
+                           (const double2 *)in, count, scale);
+    return hipGetLastError();
+}
+
+hipError_t pointwise_mul(int dtype, void *y, const void *x, const void *h, int64_t count,
+                         int64_t hcount, int conj_h, double scale, hipStream_t s)
+{
+    const int64_t g = gs_grid(count);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_pointwise_mul<float>, dim3(g), dim3(256), 0, s, (float2 *)y,
+                           (const float2 *)x, (const float2 *)h, count, hcount, conj_h, (float)scale);
+    else
+        hipLaunchKernelGGL(afft_pointwise_mul<double>, dim3(g), dim3(256), 0, s, (double2 *)y,
+                           (const double2 *)x, (const double2 *)h, count, hcount, conj_h, scale);
+    return hipGetLastError();
+}
+
+} // namespace afft

@@ -1,0 +1,811 @@
+// afft_plan.cpp -- planning, the plan cache and the C ABI of the FFT engine.
+//
+// A plan is an immutable list of kernel launches plus the device tables they read (pass
+// twiddles, four-step twiddles, Bluestein chirps and filters, all generated on the device in
+// FP64 and rounded to the element type).  It holds no stream and no mutable state, so several
+// host threads may execute one plan on different streams at once; this is the fix for the
+// shared-handle setStream/exec race the cuFFT plugin leaves open (LLVM/PTX.hs:121-124,
+// PTX/Plans.hs:58-63).  The cache is keyed on the full (device, dtype, rank, axes, extents)
+// tuple rather than a hash of it (PTX/Plans.hs:72 keys on (context, hash)).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/afft.h"
+#include "afft_internal.hpp"
+#include "afft_plan.hpp"
+
+namespace afft {
+
+// ----------------------------------------------------------------------------------------------
+// Factorisation (host only).
+
+static const int kRadices[] = {16, 8, 5, 4, 3, 2};
+
+static bool is_5smooth(int64_t n)
+{
+    if (n < 1) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+static int prime_mask(int64_t n)
+{
+    int m = 0;
+    if (n % 2 == 0) m |= 1;
+    if (n % 3 == 0) m |= 2;
+    if (n % 5 == 0) m |= 4;
+    return m;
+}
+
+int64_t lds_limit_elems(int dtype, size_t lds_bytes)
+{
+    // largest n with lds_padded_len(n) elements fitting the LDS budget, rounded down to a
+    // 5-smooth-friendly bound: 16384 (c64) / 8192 (c128) on a 160 KiB LDS.
+    const size_t esz = dtype == AFFT_C64 ? 8 : 16;
+    int64_t n = 1;
+    while ((size_t)lds_padded_len((int)(n * 2)) * esz <= lds_bytes) n *= 2;
+    return n;
+}
+
+bool choose_layout(int64_t n, int dtype, LdsLayout &out)
+{
+    if (n < 1 || n > (1 << 30)) return false;
+    if (n == 1) {
+        out.E = 1;
+        out.npass = 0;
+        return true;
+    }
+    if (!is_5smooth(n)) return false;
+    const int target = 16;
+    bool found = false;
+    LdsLayout best{};
+    int best_score = 1 << 30;
+    const int need = prime_mask(n);
+    for (int ie = 0; ie < kNumElemsPerThread; ++ie) {
+        const int E = kElemsPerThread[ie];
+        if (E > n || n % E != 0) continue;
+        if ((prime_mask(E) & need) != need) continue;
+        const int64_t T = n / E;
+        if (T > max_block_threads(dtype == AFFT_C128, E)) continue;
+        if (dtype == AFFT_C128 && E > 32) continue; // FP64 data beyond 128 VGPRs
+        LdsLayout L{};
+        L.E = E;
+        int64_t rem = n;
+        int np = 0;
+        bool ok = true;
+        while (rem > 1) {
+            int pick = 0;
+            for (int R : kRadices)
+                if (E % R == 0 && rem % R == 0) {
+                    pick = R;
+                    break;
+                }
+            if (!pick || np >= kMaxPasses) {
+                ok = false;
+                break;
+            }
+            L.radix[np++] = pick;
+            rem /= pick;
+        }
+        if (!ok) continue;
+        L.npass = np;
+        const int score = np * 1000 + std::abs(E - target);
+        if (score < best_score) {
+            best_score = score;
+            best = L;
+            found = true;
+        }
+    }
+    if (found) out = best;
+    return found;
+}
+
+// Bluestein padded length: the smallest M >= 2n - 1 of the form 2^k, 3*2^k or 5*2^k.
+int64_t bluestein_length(int64_t n)
+{
+    const int64_t need = 2 * n - 1;
+    int64_t best = 0;
+    for (int64_t base : {1, 3, 5}) {
+        int64_t m = base;
+        while (m < need) m *= 2;
+        if (best == 0 || m < best) best = m;
+    }
+    return best;
+}
+
+static PassTable make_pass_table(int64_t n, const LdsLayout &L, int64_t *tw_elems)
+{
+    PassTable pt{};
+    pt.n = (int)n;
+    pt.npass = L.npass;
+    int64_t off = 0;
+    int64_t Ns = 1;
+    for (int p = 0; p < L.npass; ++p) {
+        pt.radix[p] = L.radix[p];
+        pt.tw_off[p] = (int)off;
+        off += Ns * L.radix[p];
+        Ns *= L.radix[p];
+    }
+    *tw_elems = off;
+    return pt;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Plan construction.
+
+namespace {
+
+struct Builder {
+    Plan &plan;
+    const DeviceInfo &dev;
+    hipStream_t s;
+    int status = AFFT_SUCCESS;
+
+    size_t esz() const { return plan.dtype == AFFT_C64 ? 8 : 16; }
+
+    void *alloc(int64_t bytes)
+    {
+        if (bytes <= 0) return nullptr;
+        void *p = nullptr;
+        if (hipMalloc(&p, (size_t)bytes) != hipSuccess) {
+            status = AFFT_ERROR_ALLOC;
+            return nullptr;
+        }
+        plan.allocs.push_back(p);
+        plan.device_bytes += bytes;
+        return p;
+    }
+
+    bool check(hipError_t e)
+    {
+        if (e != hipSuccess && status == AFFT_SUCCESS) status = AFFT_ERROR_HIP;
+        return e == hipSuccess;
+    }
+
+    // pass twiddles for length n with layout L (cached per length inside the plan)
+    const void *twiddles(int64_t n, const LdsLayout &L, PassTable &pt)
+    {
+        int64_t cnt = 0;
+        pt = make_pass_table(n, L, &cnt);
+        auto it = plan.tw_cache.find(n);
+        if (it != plan.tw_cache.end()) return it->second;
+        void *buf = alloc(std::max<int64_t>(cnt, 1) * (int64_t)esz());
+        if (!buf) return nullptr;
+        if (cnt > 0) check(gen_pass_twiddles(plan.dtype, buf, pt, s));
+        plan.tw_cache[n] = buf;
+        return buf;
+    }
+
+    // workgroup shape for a transform of length n with T threads, strided or not
+    void shape(int64_t n, int E, bool strided, int npass, int64_t ntrans, int &C, int &block, size_t &lds)
+    {
+        const int T = (int)(n / E);
+        const int maxb = max_block_threads(plan.dtype == AFFT_C128, E);
+        const size_t per = (size_t)lds_padded_len((int)n) * esz();
+        if (!strided) {
+            C = std::max(1, std::min(256, maxb) / T);
+            if (npass > 1)
+                while (C > 1 && (size_t)C * per > 64 * 1024) C /= 2;
+        } else {
+            C = 16;
+            while (C > 1 && (size_t)C * T > maxb) C /= 2;
+            if (npass > 1)
+                while (C > 1 && (size_t)C * per > dev.lds_bytes) C /= 2;
+            // prefer at least 64 threads per workgroup when the LDS allows it
+        }
+        if ((int64_t)C > ntrans && ntrans > 0) C = (int)std::max<int64_t>(1, ntrans);
+        block = C * T;
+        lds = npass > 1 ? (size_t)C * per : 0;
+    }
+
+    // Plain single-workgroup transform (n 5-smooth and within the LDS limit).
+    void add_stockham(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, bool strided,
+                      Slot src, Slot dst, int64_t post_L = 0, int post_level = 0,
+                      const void *post_hi = nullptr, const void *post_lo = nullptr, int post_shift = 0)
+    {
+        LdsLayout L{};
+        if (!choose_layout(n, plan.dtype, L)) {
+            status = AFFT_ERROR_UNSUPPORTED_SIZE;
+            return;
+        }
+        PassTable pt{};
+        const void *tw = twiddles(n, L, pt);
+        if (status) return;
+        Launch l{};
+        l.kind = Launch::kStockham;
+        l.src = src;
+        l.dst = dst;
+        FftArgs &a = l.fa;
+        a.tw = tw;
+        a.ntrans = ntrans;
+        a.ain = ain;
+        a.aout = aout;
+        a.pt = pt;
+        a.E = L.E;
+        shape(n, L.E, strided, L.npass, ntrans, a.C, l.block, l.lds);
+        a.lane_col = strided ? 1 : 0;
+        a.lds_interleave = strided ? 1 : 0;
+        a.scale = 1.0;
+        a.post_L = post_L;
+        a.post_level = post_level;
+        a.post_hi = post_hi;
+        a.post_lo = post_lo;
+        a.post_shift = post_shift;
+        l.grid = (ntrans + a.C - 1) / a.C;
+        plan.launches.push_back(l);
+    }
+
+    void add_bluestein(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, bool strided,
+                       Slot src, Slot dst)
+    {
+        const int64_t M = bluestein_length(n);
+        LdsLayout L{};
+        if (M > dev.max_lds_len[plan.dtype] || !choose_layout(M, plan.dtype, L)) {
+            status = AFFT_ERROR_UNSUPPORTED_SIZE;
+            return;
+        }
+        PassTable pt{};
+        const void *tw = twiddles(M, L, pt);
+        if (status) return;
+        // chirp and filter, cached per n
+        auto it = plan.blue_cache.find(n);
+        const void *chirp = nullptr, *filt = nullptr;
+        if (it != plan.blue_cache.end()) {
+            chirp = it->second.first;
+            filt = it->second.second;
+        } else {
+            void *cbuf = alloc(n * (int64_t)esz());
+            void *fbuf = alloc(M * (int64_t)esz());
+            if (status) return;
+            check(gen_chirp(plan.dtype, cbuf, n, s));
+            // b in FP64, its FP64 FFT by a sub-plan of this engine, scaled by 1/M and rounded
+            void *b64 = nullptr;
+            if (hipMalloc(&b64, (size_t)M * 16) != hipSuccess) {
+                status = AFFT_ERROR_ALLOC;
+                return;
+            }
+            check(gen_bluestein_kernel_f64(b64, n, M, s));
+            Plan sub;
+            int64_t ext[1] = {M};
+            int st = build_plan(sub, AFFT_C128, 1, ext, AFFT_AXES_INNERMOST, plan.device, dev, s);
+            if (st == AFFT_SUCCESS) st = exec_plan(sub, -1, b64, b64, 1.0, s);
+            if (st != AFFT_SUCCESS) status = st;
+            check(convert_scale_from_f64(plan.dtype, fbuf, b64, M, 1.0 / (double)M, s));
+            check(hipStreamSynchronize(s));
+            (void)hipFree(b64);
+            free_plan(sub);
+            if (status) return;
+            plan.blue_cache[n] = {cbuf, fbuf};
+            chirp = cbuf;
+            filt = fbuf;
+        }
+        Launch l{};
+        l.kind = Launch::kBluestein;
+        l.src = src;
+        l.dst = dst;
+        BluesteinArgs &a = l.ba;
+        a.tw = tw;
+        a.chirp = chirp;
+        a.filt = filt;
+        a.ntrans = ntrans;
+        a.n = n;
+        a.ain = ain;
+        a.aout = aout;
+        a.pt = pt;
+        a.E = L.E;
+        int C;
+        shape(M, L.E, strided, L.npass, ntrans, C, l.block, l.lds);
+        a.C = C;
+        a.lane_col = strided ? 1 : 0;
+        a.lds_interleave = strided ? 1 : 0;
+        a.scale = 1.0;
+        l.grid = (ntrans + C - 1) / C;
+        plan.launches.push_back(l);
+    }
+
+    // Transform along one axis: extent n, element stride s_ (product of the inner extents),
+    // o outer blocks; reads slot src, writes slot dst.
+    void add_axis(int64_t n, int64_t s_, int64_t o, Slot src, Slot dst)
+    {
+        if (status) return;
+        const bool strided = s_ > 1;
+        const int64_t ntrans = o * s_;
+        Addr a{};
+        a.cnt0 = s_;
+        a.cnt1 = 1;
+        a.st0 = 1;
+        a.st1 = 0;
+        a.st2 = n * s_;
+        a.es = s_;
+        const int64_t limit = dev.max_lds_len[plan.dtype];
+        if (n <= limit && is_5smooth(n)) {
+            add_stockham(n, ntrans, a, a, strided, src, dst);
+            return;
+        }
+        if (!is_5smooth(n)) {
+            add_bluestein(n, ntrans, a, a, strided, src, dst);
+            return;
+        }
+        // four-step: n = n1 * n2, element i = i1*n2 + i2
+        int64_t n1 = 0, n2 = 0;
+        double best = 1e300;
+        for (int64_t d = 2; d <= limit; ++d) {
+            if (n % d) continue;
+            const int64_t e = n / d;
+            if (e > limit || !is_5smooth(d) || !is_5smooth(e)) continue;
+            const double sc = std::fabs(std::log((double)d) - std::log((double)e));
+            if (sc < best) {
+                best = sc;
+                n1 = d;
+                n2 = e;
+            }
+        }
+        if (!n1) {
+            status = AFFT_ERROR_UNSUPPORTED_SIZE;
+            return;
+        }
+        // two-level table for W_n^e: lo[e & (2^sh - 1)], hi[e >> sh]
+        int sh = 0;
+        while (((int64_t)1 << (2 * sh)) < n) ++sh;
+        const int64_t lo_cnt = (int64_t)1 << sh;
+        const int64_t hi_cnt = (n + lo_cnt - 1) / lo_cnt;
+        void *lo = alloc(lo_cnt * (int64_t)esz());
+        void *hi = alloc(hi_cnt * (int64_t)esz());
+        if (status) return;
+        check(gen_power_table(plan.dtype, lo, n, 1, lo_cnt, s));
+        check(gen_power_table(plan.dtype, hi, n, lo_cnt, hi_cnt, s));
+        // pass A: over i1 (stride n2*s_) for each (c, i2, o); twiddle W_n^{i2 k1}; write
+        // transposed to (i2*n1 + k1)*s_ + c
+        Addr ia{}, oa{};
+        ia.cnt0 = s_;
+        ia.cnt1 = n2;
+        ia.st0 = 1;
+        ia.st1 = s_;
+        ia.st2 = n * s_;
+        ia.es = n2 * s_;
+        oa = ia;
+        oa.st1 = n1 * s_;
+        oa.es = s_;
+        const Slot mid = (src == dst) ? Slot::kWork : dst;
+        if (mid == Slot::kWork) plan.work_elems = std::max(plan.work_elems, o * n * s_);
+        add_stockham(n1, o * s_ * n2, ia, oa, true, src, mid, n, 1, hi, lo, sh);
+        // pass B: over i2 (stride n1*s_) for each (c, k1, o), positions unchanged
+        Addr b{};
+        b.cnt0 = s_;
+        b.cnt1 = n1;
+        b.st0 = 1;
+        b.st1 = s_;
+        b.st2 = n * s_;
+        b.es = n1 * s_;
+        add_stockham(n2, o * s_ * n1, b, b, true, mid, dst);
+    }
+};
+
+} // namespace
+
+int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes, int device,
+               const DeviceInfo &dev, hipStream_t s)
+{
+    plan.dtype = dtype;
+    plan.rank = rank;
+    plan.axes = axes;
+    plan.device = device;
+    plan.extents.assign(extents, extents + rank);
+    plan.total = 1;
+    for (int i = 0; i < rank; ++i) plan.total *= extents[i];
+    Builder b{plan, dev, s};
+    if (plan.total == 0) return AFFT_SUCCESS;
+    // innermost axis first (reads the input), the others in place on the output
+    Slot src = Slot::kIn;
+    int64_t inner = 1;
+    for (int ax = rank - 1; ax >= 0; --ax) {
+        const bool take = (axes == AFFT_AXES_ALL) || ax == rank - 1;
+        const int64_t n = extents[ax];
+        if (take && n > 1) {
+            const int64_t o = plan.total / (n * inner);
+            b.add_axis(n, inner, o, src, Slot::kOut);
+            src = Slot::kOut;
+            if (b.status) break;
+        }
+        inner *= n;
+    }
+    if (b.status == AFFT_SUCCESS && plan.launches.empty()) {
+        // every transformed extent is 1: the transform is the identity (Adhoc.hs:45); a copy
+        // kernel still applies the scale
+        Addr a{};
+        a.cnt0 = 1;
+        a.cnt1 = 1;
+        a.st2 = 1;
+        a.es = 1;
+        b.add_stockham(1, plan.total, a, a, false, Slot::kIn, Slot::kOut);
+    }
+    if (b.status == AFFT_SUCCESS && plan.work_elems > 0) {
+        plan.work = b.alloc(plan.work_elems * (int64_t)b.esz());
+    }
+    if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
+    // some four-step passes transpose: in-place execution needs a private copy of the input
+    for (const Launch &l : plan.launches)
+        if (l.src == Slot::kIn && l.kind == Launch::kStockham && l.fa.post_L) plan.inplace_needs_copy = true;
+    return b.status;
+}
+
+void free_plan(Plan &plan)
+{
+    for (void *p : plan.allocs) (void)hipFree(p);
+    plan.allocs.clear();
+    plan.launches.clear();
+    plan.tw_cache.clear();
+    plan.blue_cache.clear();
+}
+
+static void *slot_ptr(Slot s, const void *in, void *out, void *work)
+{
+    switch (s) {
+    case Slot::kIn: return const_cast<void *>(in);
+    case Slot::kOut: return out;
+    case Slot::kWork: return work;
+    }
+    return nullptr;
+}
+
+int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
+              hipStream_t s)
+{
+    if (plan.total == 0) return AFFT_SUCCESS;
+    const void *src_in = d_in;
+    void *tmp = nullptr;
+    if (d_in == d_out && plan.inplace_needs_copy) {
+        const size_t bytes = (size_t)plan.total * (plan.dtype == AFFT_C64 ? 8 : 16);
+        if (hipMallocAsync(&tmp, bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
+        if (hipMemcpyAsync(tmp, d_in, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            (void)hipFreeAsync(tmp, s);
+            return AFFT_ERROR_HIP;
+        }
+        src_in = tmp;
+    }
+    const int conj = direction > 0 ? 1 : 0;
+    const size_t n = plan.launches.size();
+    int st = AFFT_SUCCESS;
+    for (size_t i = 0; i < n && st == AFFT_SUCCESS; ++i) {
+        const Launch &l = plan.launches[i];
+        const bool first = i == 0, last = i + 1 == n;
+        hipError_t e;
+        if (l.kind == Launch::kStockham) {
+            FftArgs a = l.fa;
+            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.conj_in = first ? conj : 0;
+            a.conj_out = last ? conj : 0;
+            a.scale = last ? scale : 1.0;
+            e = launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
+        } else {
+            BluesteinArgs a = l.ba;
+            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.conj_in = first ? conj : 0;
+            a.conj_out = last ? conj : 0;
+            a.scale = last ? scale : 1.0;
+            e = launch_bluestein(plan.dtype, a, l.block, l.grid, l.lds, s);
+        }
+        if (e != hipSuccess) st = AFFT_ERROR_HIP;
+    }
+    if (tmp) (void)hipFreeAsync(tmp, s);
+    return st;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Device info and the plan cache.
+
+static std::mutex g_dev_mu;
+static std::map<int, DeviceInfo> g_dev;
+
+int device_info(int device, DeviceInfo &out)
+{
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto it = g_dev.find(device);
+    if (it != g_dev.end()) {
+        out = it->second;
+        return AFFT_SUCCESS;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return AFFT_ERROR_NO_DEVICE;
+    if (device < 0 || device >= count) return AFFT_ERROR_INVALID_VALUE;
+    DeviceInfo d{};
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return AFFT_ERROR_HIP;
+    d.lds_bytes = lds > 0 ? (size_t)lds : 65536;
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    d.cus = cus;
+    // leave room for the padding of the largest power of two that fits
+    d.max_lds_len[AFFT_C64] = lds_limit_elems(AFFT_C64, d.lds_bytes);
+    d.max_lds_len[AFFT_C128] = lds_limit_elems(AFFT_C128, d.lds_bytes);
+    g_dev[device] = d;
+    out = d;
+    return AFFT_SUCCESS;
+}
+
+struct DeviceGuard {
+    int old = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&old) != hipSuccess) return;
+        ok = old == dev || hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (ok && old >= 0) (void)hipSetDevice(old);
+    }
+};
+
+int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device)
+{
+    DeviceInfo dev;
+    int st = device_info(device, dev);
+    if (st) return st;
+    DeviceGuard g(device);
+    if (!g.ok) return AFFT_ERROR_HIP;
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return AFFT_ERROR_HIP;
+    Plan *p = new (std::nothrow) Plan();
+    if (!p) {
+        (void)hipStreamDestroy(s);
+        return AFFT_ERROR_ALLOC;
+    }
+    st = build_plan(*p, dtype, rank, extents, axes, device, dev, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    if (st) {
+        free_plan(*p);
+        delete p;
+        return st;
+    }
+    out = p;
+    return AFFT_SUCCESS;
+}
+
+using Key = std::tuple<int, int, int, int, std::vector<int64_t>>;
+static std::mutex g_cache_mu;
+static std::map<Key, Plan *> g_cache;
+
+int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device)
+{
+    Key k{device, dtype, rank, axes, std::vector<int64_t>(extents, extents + rank)};
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache.find(k);
+    if (it != g_cache.end()) {
+        out = it->second;
+        return AFFT_SUCCESS;
+    }
+    Plan *p = nullptr;
+    int st = create_plan(p, dtype, rank, extents, axes, device);
+    if (st) return st;
+    g_cache[k] = p;
+    out = p;
+    return AFFT_SUCCESS;
+}
+
+void clear_cache()
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto &kv : g_cache) {
+        int old = -1;
+        (void)hipGetDevice(&old);
+        (void)hipSetDevice(kv.second->device);
+        (void)hipDeviceSynchronize();
+        free_plan(*kv.second);
+        delete kv.second;
+        if (old >= 0) (void)hipSetDevice(old);
+    }
+    g_cache.clear();
+}
+
+int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream)
+{
+    DeviceGuard g(p.device);
+    if (!g.ok) return AFFT_ERROR_HIP;
+    return exec_plan(p, direction, in, out, scale, (hipStream_t)stream);
+}
+
+int current_device(int &dev)
+{
+    if (hipGetDevice(&dev) != hipSuccess) return AFFT_ERROR_NO_DEVICE;
+    return AFFT_SUCCESS;
+}
+
+} // namespace afft
+
+// ==============================================================================================
+// C ABI
+
+using namespace afft;
+
+static bool valid_dtype(int d) { return d == AFFT_C64 || d == AFFT_C128; }
+static bool valid_mode(int m) { return m == AFFT_FORWARD || m == AFFT_REVERSE || m == AFFT_INVERSE; }
+
+extern "C" const char *afft_strerror(int status)
+{
+    switch (status) {
+    case AFFT_SUCCESS: return "success";
+    case AFFT_ERROR_INVALID_VALUE: return "invalid argument";
+    case AFFT_ERROR_UNSUPPORTED_RANK: return "unsupported rank";
+    case AFFT_ERROR_UNSUPPORTED_DTYPE: return "unsupported element type";
+    case AFFT_ERROR_UNSUPPORTED_SIZE: return "unsupported transform length";
+    case AFFT_ERROR_ALLOC: return "allocation failed";
+    case AFFT_ERROR_HIP: return "HIP runtime error";
+    case AFFT_ERROR_NO_DEVICE: return "no usable device";
+    case AFFT_ERROR_IO: return "plan-cache file error";
+    case AFFT_ERROR_INTERNAL: return "internal error";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int afft_version(void) { return 10000; }
+
+static int check_shape(int dtype, int rank, const int64_t *extents, int max_rank)
+{
+    if (!valid_dtype(dtype)) return AFFT_ERROR_UNSUPPORTED_DTYPE;
+    if (rank < 1 || rank > max_rank) return AFFT_ERROR_UNSUPPORTED_RANK;
+    if (!extents) return AFFT_ERROR_INVALID_VALUE;
+    for (int i = 0; i < rank; ++i)
+        if (extents[i] < 0) return AFFT_ERROR_INVALID_VALUE;
+    return AFFT_SUCCESS;
+}
+
+static int64_t total_of(int rank, const int64_t *e)
+{
+    int64_t t = 1;
+    for (int i = 0; i < rank; ++i) t *= e[i];
+    return t;
+}
+
+// The front-end operation: mode semantics of FFT.hs (Inverse divides by `divisor`).
+static int front_end(int mode, int dtype, int rank, const int64_t *extents, int axes, int64_t divisor,
+                     const void *in, void *out, void *stream)
+{
+    if (!valid_mode(mode)) return AFFT_ERROR_INVALID_VALUE;
+    int st = check_shape(dtype, rank, extents, 8);
+    if (st) return st;
+    if (total_of(rank, extents) == 0) return AFFT_SUCCESS;
+    if (!in || !out) return AFFT_ERROR_INVALID_VALUE;
+    int dev;
+    if ((st = current_device(dev))) return st;
+    const Plan *p = nullptr;
+    if ((st = cached_plan(p, dtype, rank, extents, axes, dev))) return st;
+    const int dir = mode == AFFT_FORWARD ? -1 : 1;
+    const double scale = (mode == AFFT_INVERSE && divisor > 0) ? 1.0 / (double)divisor : 1.0;
+    return run_plan(*p, dir, in, out, scale, stream);
+}
+
+extern "C" int afft_fft(int mode, int dtype, int rank, const int64_t *extents, const void *d_in,
+                        void *d_out, void *stream)
+{
+    int st = check_shape(dtype, rank, extents, 8);
+    if (st) return st;
+    return front_end(mode, dtype, rank, extents, AFFT_AXES_INNERMOST, extents[rank - 1], d_in, d_out, stream);
+}
+
+extern "C" int afft_fft1D(int mode, int dtype, int64_t n, const void *d_in, void *d_out, void *stream)
+{
+    const int64_t e[1] = {n};
+    return front_end(mode, dtype, 1, e, AFFT_AXES_ALL, n, d_in, d_out, stream);
+}
+
+extern "C" int afft_fft2D(int mode, int dtype, int64_t h, int64_t w, const void *d_in, void *d_out,
+                          void *stream)
+{
+    const int64_t e[2] = {h, w};
+    return front_end(mode, dtype, 2, e, AFFT_AXES_ALL, h * w, d_in, d_out, stream);
+}
+
+extern "C" int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, const void *d_in,
+                          void *d_out, void *stream)
+{
+    const int64_t e[3] = {d, h, w};
+    return front_end(mode, dtype, 3, e, AFFT_AXES_ALL, d * h * w, d_in, d_out, stream);
+}
+
+extern "C" int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
+                                int device)
+{
+    if (!plan) return AFFT_ERROR_INVALID_VALUE;
+    *plan = nullptr;
+    int st = check_shape(dtype, rank, extents, 8);
+    if (st) return st;
+    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL) return AFFT_ERROR_INVALID_VALUE;
+    Plan *p = nullptr;
+    st = create_plan(p, dtype, rank, extents, axes, device);
+    if (st) return st;
+    *plan = reinterpret_cast<afft_plan *>(p);
+    return AFFT_SUCCESS;
+}
+
+extern "C" int afft_plan_cached(const afft_plan **plan, int dtype, int rank, const int64_t *extents,
+                                int axes, int device)
+{
+    if (!plan) return AFFT_ERROR_INVALID_VALUE;
+    *plan = nullptr;
+    int st = check_shape(dtype, rank, extents, 8);
+    if (st) return st;
+    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL) return AFFT_ERROR_INVALID_VALUE;
+    const Plan *p = nullptr;
+    st = cached_plan(p, dtype, rank, extents, axes, device);
+    if (st) return st;
+    *plan = reinterpret_cast<const afft_plan *>(p);
+    return AFFT_SUCCESS;
+}
+
+extern "C" int afft_exec(const afft_plan *plan, int direction, const void *d_in, void *d_out,
+                         double scale, void *stream)
+{
+    if (!plan || (direction != -1 && direction != 1)) return AFFT_ERROR_INVALID_VALUE;
+    const Plan *p = reinterpret_cast<const Plan *>(plan);
+    if (p->total == 0) return AFFT_SUCCESS;
+    if (!d_in || !d_out) return AFFT_ERROR_INVALID_VALUE;
+    return run_plan(*p, direction, d_in, d_out, scale, stream);
+}
+
+extern "C" int afft_plan_destroy(afft_plan *plan)
+{
+    if (!plan) return AFFT_ERROR_INVALID_VALUE;
+    Plan *p = reinterpret_cast<Plan *>(plan);
+    {
+        DeviceGuard g(p->device);
+        (void)hipDeviceSynchronize();
+        free_plan(*p);
+    }
+    delete p;
+    return AFFT_SUCCESS;
+}
+
+extern "C" int afft_cache_clear(void)
+{
+    clear_cache();
+    return AFFT_SUCCESS;
+}
+
+extern "C" int afft_plan_info(const afft_plan *plan, int *launches, int64_t *device_bytes)
+{
+    if (!plan) return AFFT_ERROR_INVALID_VALUE;
+    const Plan *p = reinterpret_cast<const Plan *>(plan);
+    if (launches) *launches = (int)p->launches.size();
+    if (device_bytes) *device_bytes = p->device_bytes;
+    return AFFT_SUCCESS;
+}
+
+extern "C" int afft_describe(int64_t n, int dtype, int *radices, int max_passes, int *n_passes,
+                             int *elems_per_thread, int64_t *bluestein_len)
+{
+    if (!valid_dtype(dtype)) return AFFT_ERROR_UNSUPPORTED_DTYPE;
+    if (n < 1) return AFFT_ERROR_INVALID_VALUE;
+    // the single-workgroup limit on a 160 KiB LDS (gfx950)
+    const int64_t limit = lds_limit_elems(dtype, 160 * 1024);
+    int64_t len = n, blue = 0;
+    if (!is_5smooth(n)) {
+        blue = bluestein_length(n);
+        len = blue;
+    }
+    if (len > limit) return AFFT_ERROR_UNSUPPORTED_SIZE;
+    LdsLayout L{};
+    if (!choose_layout(len, dtype, L)) return AFFT_ERROR_UNSUPPORTED_SIZE;
+    if (n_passes) *n_passes = L.npass;
+    if (elems_per_thread) *elems_per_thread = L.E;
+    if (bluestein_len) *bluestein_len = blue;
+    if (radices)
+        for (int i = 0; i < L.npass && i < max_passes; ++i) radices[i] = L.radix[i];
+    return AFFT_SUCCESS;
+}

@@ -1,0 +1,69 @@
+// afft_plan.hpp -- the plan representation shared by the planner and the extension entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "afft_internal.hpp"
+
+namespace afft {
+
+// Layout of one single-workgroup transform: E elements per thread and the pass radices.
+struct LdsLayout {
+    int E;
+    int npass;
+    int radix[kMaxPasses];
+};
+
+struct DeviceInfo {
+    size_t lds_bytes;        // LDS a workgroup may allocate
+    int cus;                 // compute units
+    int64_t max_lds_len[2];  // longest single-workgroup transform per dtype
+};
+
+enum class Slot { kIn, kOut, kWork };
+
+struct Launch {
+    enum Kind { kStockham, kBluestein } kind;
+    Slot src, dst;
+    FftArgs fa;
+    BluesteinArgs ba;
+    int block;
+    int64_t grid;
+    size_t lds;
+};
+
+struct Plan {
+    int dtype = 0, rank = 0, axes = 0, device = 0;
+    std::vector<int64_t> extents;
+    int64_t total = 0;
+    std::vector<Launch> launches;
+    std::vector<void *> allocs;
+    int64_t device_bytes = 0;
+    std::map<int64_t, const void *> tw_cache;
+    std::map<int64_t, std::pair<const void *, const void *>> blue_cache;
+    int64_t work_elems = 0;
+    void *work = nullptr;
+    bool inplace_needs_copy = false;
+};
+
+bool choose_layout(int64_t n, int dtype, LdsLayout &out);
+int64_t bluestein_length(int64_t n);
+int64_t lds_limit_elems(int dtype, size_t lds_bytes);
+int device_info(int device, DeviceInfo &out);
+int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes, int device,
+               const DeviceInfo &dev, hipStream_t s);
+void free_plan(Plan &plan);
+int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
+              hipStream_t s);
+int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
+int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
+int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream);
+int current_device(int &dev);
+void clear_cache();
+
+} // namespace afft

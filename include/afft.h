@@ -1,0 +1,185 @@
+/*
+ * afft.h -- C ABI of the MI355X (gfx950) FFT engine behind Data.Array.Accelerate.Math.FFT.
+ *
+ * Everything here is plain C: device pointers are `void *`, sizes are int64_t, the HIP stream is
+ * an opaque `void *` (a hipStream_t, NULL = the default stream).  Every function returns an
+ * `int` status (AFFT_SUCCESS = 0, see afft_strerror) and nothing throws across this boundary.
+ *
+ * Layout contract (src/Data/Array/Accelerate/Math/FFT/Type.hs:28-39): an element of
+ * `Complex Float` / `Complex Double` is an interleaved (re, im) pair (Vec2 e), 8 or 16 bytes;
+ * arrays are row-major in Accelerate's shape order Z :. e_0 :. ... :. e_{rank-1}, the innermost
+ * extent e_{rank-1} varying fastest.  `extents` arguments list e_0 .. e_{rank-1}.
+ *
+ * Semantics of `mode` (src/Data/Array/Accelerate/Math/FFT/Mode.hs:15-26):
+ *   AFFT_FORWARD  sum_n x[n] exp(-2 pi i nk/N)
+ *   AFFT_REVERSE  sum_n x[n] exp(+2 pi i nk/N), unnormalised
+ *   AFFT_INVERSE  as Reverse, then divided by N, fused into the last pass.  N is the innermost
+ *                 extent for afft_fft (FFT.hs:69), the length for afft_fft1D (FFT.hs:98) and the
+ *                 total size for afft_fft2D/afft_fft3D (FFT.hs:125,155).
+ *
+ * All transforms are out-of-place (in-place when d_in == d_out is also allowed) and leave the
+ * input unmodified when out-of-place, as the reference plugins do (LLVM/PTX.hs:92 writes a fresh
+ * array; LLVM/Native/Base.hs:47 asks FFTW to preserve the input).  Calls are asynchronous on
+ * the given stream, like the cuFFT path they replace (LLVM/PTX.hs:93,121-124).
+ */
+#ifndef AFFT_H
+#define AFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define AFFT_SUCCESS 0
+#define AFFT_ERROR_INVALID_VALUE 1    /* null pointer, negative extent, bad mode/axes value  */
+#define AFFT_ERROR_UNSUPPORTED_RANK 2 /* rank outside the entry point's range                */
+#define AFFT_ERROR_UNSUPPORTED_DTYPE 3
+#define AFFT_ERROR_UNSUPPORTED_SIZE 4 /* a length the engine cannot plan                      */
+#define AFFT_ERROR_ALLOC 5            /* device or host allocation failed                     */
+#define AFFT_ERROR_HIP 6              /* a HIP runtime call failed                            */
+#define AFFT_ERROR_NO_DEVICE 7        /* no usable gfx950 device                              */
+#define AFFT_ERROR_IO 8               /* plan-cache file could not be read or written        */
+#define AFFT_ERROR_INTERNAL 9
+
+/* ---- enums ------------------------------------------------------------------------------ */
+/* element type: NumericRfloat32 / NumericRfloat64 (FFT/Type.hs:28-30) */
+#define AFFT_C64 0  /* Complex Float  (cuFFT C2C, LLVM/PTX.hs:126-127) */
+#define AFFT_C128 1 /* Complex Double (cuFFT Z2Z, LLVM/PTX.hs:128)     */
+
+/* Mode (FFT/Mode.hs:15-19) */
+#define AFFT_FORWARD 0
+#define AFFT_REVERSE 1
+#define AFFT_INVERSE 2
+
+/* which axes a plan transforms */
+#define AFFT_AXES_INNERMOST 0 /* fft: innermost axis, outer axes are a batch */
+#define AFFT_AXES_ALL 1       /* fft1D / fft2D / fft3D: every axis           */
+
+typedef struct afft_plan afft_plan;
+typedef struct afft_ragged_plan afft_ragged_plan;
+
+/* ---- library ---------------------------------------------------------------------------- */
+
+/* Human-readable text for a status code; never NULL. */
+const char *afft_strerror(int status);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int afft_version(void);
+
+/* ---- the four front-end operations (the plugin entry points) ----------------------------- */
+/*
+ * Each call plans on first use of a (device, dtype, shape, axes) key and reuses the plan from a
+ * mutex-guarded cache keyed on the full tuple afterwards.
+ */
+
+/* fft: DFT along the innermost axis of an array of rank 1..8; the outer axes are a batch.
+ * Replaces LLVM/PTX.hs:52-61 (cuFFT planMany over the innermost extent, rank <= 3) and
+ * LLVM/Native.hs:49-72 (FFTW dftGU over axis [rank-1], rank <= 5); the front-end is FFT.hs:63-84. */
+int afft_fft(int mode, int dtype, int rank, const int64_t *extents, const void *d_in, void *d_out,
+             void *stream);
+
+/* fft1D: DFT of a vector of length n.
+ * Replaces LLVM/PTX.hs:63-64 (plan1D, :141) and LLVM/Native.hs:74-82; front-end FFT.hs:92-111. */
+int afft_fft1D(int mode, int dtype, int64_t n, const void *d_in, void *d_out, void *stream);
+
+/* fft2D: separable DFT over both axes of an h x w matrix (w innermost).
+ * Replaces LLVM/PTX.hs:66-67 (plan2D h w, :148) and LLVM/Native.hs:84-92; front-end FFT.hs:119-142. */
+int afft_fft2D(int mode, int dtype, int64_t h, int64_t w, const void *d_in, void *d_out,
+               void *stream);
+
+/* fft3D: DFT over all axes of a d x h x w array (w innermost).
+ * Replaces LLVM/PTX.hs:69-70 (plan3D d h w, :155) and LLVM/Native.hs:94-102; front-end FFT.hs:150-173. */
+int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, const void *d_in, void *d_out,
+               void *stream);
+
+/* ---- explicit plans --------------------------------------------------------------------- */
+
+/* Build an immutable plan: kernels chosen, twiddles generated on the device in FP64 and rounded
+ * to the element type.  `device` is a HIP device ordinal.  Plans carry no stream state, so one
+ * plan may be executed concurrently on several streams.
+ * Replaces the cuFFT handle creation of LLVM/PTX.hs:137-170 and PTX/Plans.hs:40-90. */
+int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
+                     int device);
+
+/* The shared cached plan for a key (what afft_fft* use).  The pointer stays valid until
+ * afft_cache_clear; do not destroy it.  Replaces PTX/Plans.hs:66-86 (withPlan). */
+int afft_plan_cached(const afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
+                     int device);
+
+/* Run a plan.  direction: -1 (Forward) or +1 (Reverse), unnormalised, then every output element
+ * is multiplied by `scale` inside the last pass (1.0 = the reference plugin contract; 1/N gives
+ * the normalised Inverse).  Replaces cuFFT's execC2C/execZ2Z (LLVM/PTX.hs:112-132). */
+int afft_exec(const afft_plan *plan, int direction, const void *d_in, void *d_out, double scale,
+              void *stream);
+
+int afft_plan_destroy(afft_plan *plan);
+
+/* Drop every cached plan (and its device memory).  Not safe while another thread is executing a
+ * cached plan. */
+int afft_cache_clear(void);
+
+/* Number of kernel launches one afft_exec of the plan makes and the bytes of device workspace it
+ * holds (twiddles, Bluestein filters, intermediate buffers). */
+int afft_plan_info(const afft_plan *plan, int *launches, int64_t *device_bytes);
+
+/* ---- host-only planning (no device needed) ---------------------------------------------- */
+
+/* Describe how a single-workgroup LDS transform of length n would be laid out: the radix of each
+ * Stockham pass (at most max_passes written), elements per thread, and the Bluestein padded
+ * length (0 when n is 5-smooth).  Returns AFFT_ERROR_UNSUPPORTED_SIZE when n is beyond the
+ * single-workgroup limit for the dtype (the multi-pass four-step path then applies). */
+int afft_describe(int64_t n, int dtype, int *radices, int max_passes, int *n_passes,
+                  int *elems_per_thread, int64_t *bluestein_len);
+
+/* ---- extensions (no counterpart in the reference API) ------------------------------------ */
+
+/* VJP of a front-end op with respect to its input: given the cotangent d_gy of y = op(mode, x),
+ * writes d_gx = op^H(d_gy) (the conjugate-transpose transform: Forward <-> Reverse, with the 1/N
+ * of Inverse kept).  `op` is 0 = fft (innermost), 1 = fft1D, 2 = fft2D, 3 = fft3D; `extents`
+ * has `rank` entries. */
+int afft_fft_vjp(int op, int mode, int dtype, int rank, const int64_t *extents, const void *d_gy,
+                 void *d_gx, void *stream);
+
+/* Ragged batch: `count` independent 1D transforms of the given lengths, stored back to back
+ * (row i starts at element sum_{j<i} lengths[j]).  One persistent kernel drains a device-side
+ * work queue; large rows and Bluestein rows are handled inside the same launch. */
+int afft_ragged_plan_create(afft_ragged_plan **plan, int dtype, int64_t count,
+                            const int64_t *lengths, int device);
+int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
+                     void *stream);
+/* The same work captured once into a hipGraph per (mode, d_in, d_out) and replayed. */
+int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
+                           void *stream);
+int afft_ragged_plan_destroy(afft_ragged_plan *plan);
+/* Total elements of the batch (sum of lengths). */
+int afft_ragged_total(const afft_ragged_plan *plan, int64_t *total);
+
+/* Spectral convolution over a batch of h x w images:
+ *   y_b = Inverse(fft2D(x_b) * H)  (elementwise product with the h x w filter H).
+ * Fused: row pass, then one kernel that does the column transform, the filter product and the
+ * inverse column transform in LDS, then the inverse row pass with the 1/(h w) scale. */
+int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                           const void *d_H, void *d_y, void *stream);
+
+/* Its VJP: given d_gy = dL/dy (conjugate-Wirtinger convention, as torch), writes
+ *   d_gx = (1/hw) F^H (conj(H) * F gy)        and
+ *   d_gH = (1/hw) sum_b conj(F x_b) * F gy_b
+ * recomputing F x_b instead of storing it.  Either output pointer may be NULL to skip it. */
+int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                           const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
+                           void *stream);
+
+/* Autotuning: time the candidate kernel layouts for a key on the device and remember the best in
+ * the plan cache; afft_tune_save/afft_tune_load persist the choices as a text file keyed by
+ * (gfx arch, HIP runtime version, dtype, length, layout). */
+int afft_autotune(int dtype, int rank, const int64_t *extents, int axes, int device);
+int afft_tune_save(const char *path);
+int afft_tune_load(const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFFT_H */
