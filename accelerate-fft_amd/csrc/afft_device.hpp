@@ -51,6 +51,42 @@ template <typename V> __device__ __forceinline__ V c_conj(V a) { return c_make<V
 template <typename V> __device__ __forceinline__ V c_mul_negi(V a) { return c_make<V>(a.y, -a.x); }
 template <typename V> __device__ __forceinline__ V c_scale(V a, decltype(V::x) s) { return c_make<V>(a.x * s, a.y * s); }
 
+// Streaming (non-temporal) accesses for data read or written exactly once by a plan: they do
+// not displace the intermediate arrays a multi-pass plan keeps in L2 / Infinity Cache.
+template <typename V> __device__ __forceinline__ V c_load_nt(const V *p)
+{
+    typedef decltype(V::x) Real;
+    typedef Real vec2 __attribute__((ext_vector_type(2)));
+    const vec2 v = __builtin_nontemporal_load(reinterpret_cast<const vec2 *>(p));
+    return c_make<V>(v.x, v.y);
+}
+template <typename V> __device__ __forceinline__ void c_store_nt(V *p, V a)
+{
+    typedef decltype(V::x) Real;
+    typedef Real vec2 __attribute__((ext_vector_type(2)));
+    vec2 v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<vec2 *>(p));
+}
+template <typename V> __device__ __forceinline__ V c_load(const V *p, bool nt) { return nt ? c_load_nt(p) : *p; }
+template <typename V> __device__ __forceinline__ void c_store(V *p, V a, bool nt)
+{
+    if (nt) c_store_nt(p, a);
+    else *p = a;
+}
+
+// Workgroup -> tile order that keeps consecutive tiles on one XCD.  The dispatcher hands
+// workgroup b to XCD b mod 8; tiles that share cache lines (adjacent column tiles of a strided
+// pass) must meet in the same L2, so XCD x takes the contiguous tile range [x G/8, (x+1) G/8).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t G)
+{
+    constexpr int kXcd = 8;
+    const int64_t full = (G / kXcd) * kXcd;
+    if (b >= full) return b;
+    return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
+
 // ----------------------------------------------------------------------------------------------
 // Forward DFT_R butterflies on v[0..R-1], in place.
 

@@ -40,6 +40,8 @@ struct FftArgs {
     int post_shift;
     const void *post_hi;
     const void *post_lo;
+    int xcd_remap;       // map workgroups to tiles XCD-contiguously (strided passes)
+    int nt_in, nt_out;   // streaming loads / stores (plan input read once / final output)
 };
 
 // Fused single-workgroup Bluestein (chirp-z) launch for a length n that is not 5-smooth:
@@ -58,7 +60,30 @@ struct BluesteinArgs {
     int lane_col, lds_interleave;
     int conj_in, conj_out;
     double scale;
+    int xcd_remap;
+    int nt_in, nt_out;
 };
+
+// Persistent batch of contiguous rows of a length with a compiled layout (afft_rows.hip).
+struct RowArgs {
+    const void *in;
+    void *out;
+    const void *tw;      // W_N^m, m in [0, N), element type
+    int64_t nrows;
+    int conj_in;
+    int conj_out;
+    double scale;
+    int nt_in, nt_out;
+};
+
+// afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,
+// LDS bytes, rows per tile, resident workgroups per CU, and the launcher.
+int row_variants(int dtype, int n);
+int row_block(int dtype, int n, int variant);
+size_t row_lds_bytes(int dtype, int n, int variant);
+int row_rows_per_tile(int dtype, int n, int variant);
+int row_occupancy(int dtype, int n, int variant);
+hipError_t launch_rows(int dtype, int n, int variant, const RowArgs &a, int64_t grid, hipStream_t s);
 
 // Launchers (afft_kernels.hip).  They return a hipError_t.
 hipError_t launch_stockham(int dtype, const FftArgs &a, int block, int64_t grid, size_t lds,

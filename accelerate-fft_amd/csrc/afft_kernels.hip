@@ -76,16 +76,18 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     const int T = n / E;
     int c, t;
     split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
-    const int64_t j = (int64_t)blockIdx.x * a.C + c;
+    const int64_t tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t j = tile * a.C + c;
     const bool active = (c < a.C) && (j < a.ntrans);
 
     V reg[E];
     if (active) {
         const V *in = reinterpret_cast<const V *>(a.in);
         const int64_t base = addr_base(a.ain, j);
+        const bool nt = a.nt_in;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            V v = in[base + (int64_t)(t + i * T) * a.ain.es];
+            V v = c_load(in + base + (int64_t)(t + i * T) * a.ain.es, nt);
             reg[i] = a.conj_in ? c_conj(v) : v;
         }
     } else {
@@ -112,7 +114,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
                                           reinterpret_cast<const V *>(a.post_lo), e, a.post_shift));
             }
             if (a.conj_out) v = c_conj(v);
-            out[base + (int64_t)k * a.aout.es] = c_scale(v, sc);
+            c_store(out + base + (int64_t)k * a.aout.es, c_scale(v, sc), a.nt_out);
         }
     }
 }
@@ -132,7 +134,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     const int T = M / E;
     int c, t;
     split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
-    const int64_t j = (int64_t)blockIdx.x * a.C + c;
+    const int64_t tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t j = tile * a.C + c;
     const bool active = (c < a.C) && (j < a.ntrans);
     const V *chirp = reinterpret_cast<const V *>(a.chirp);
     const V *filt = reinterpret_cast<const V *>(a.filt);
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
             const int m = t + i * T;
             V v = c_make<V>(0, 0);
             if (m < a.n) {
-                v = in[base + (int64_t)m * a.ain.es];
+                v = c_load(in + base + (int64_t)m * a.ain.es, (bool)a.nt_in);
                 if (a.conj_in) v = c_conj(v);
                 v = c_mul(v, chirp[m]);
             }
@@ -175,7 +178,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
             if (m < a.n) {
                 V v = c_mul(c_conj(reg[i]), chirp[m]);
                 if (a.conj_out) v = c_conj(v);
-                out[base + (int64_t)m * a.aout.es] = c_scale(v, sc);
+                c_store(out + base + (int64_t)m * a.aout.es, c_scale(v, sc), (bool)a.nt_out);
             }
         }
     }

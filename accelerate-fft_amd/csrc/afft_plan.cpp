@@ -32,6 +32,12 @@ namespace afft {
 
 static const int kRadices[] = {16, 8, 5, 4, 3, 2};
 
+static bool getenv_flag(const char *name)
+{
+    const char *v = std::getenv(name);
+    return v && v[0] && v[0] != '0';
+}
+
 static bool is_5smooth(int64_t n)
 {
     if (n < 1) return false;
@@ -237,6 +243,7 @@ struct Builder {
         shape(n, L.E, strided, L.npass, ntrans, a.C, l.block, l.lds);
         a.lane_col = strided ? 1 : 0;
         a.lds_interleave = strided ? 1 : 0;
+        a.xcd_remap = strided && !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
         a.scale = 1.0;
         a.post_L = post_L;
         a.post_level = post_level;
@@ -310,8 +317,44 @@ struct Builder {
         a.C = C;
         a.lane_col = strided ? 1 : 0;
         a.lds_interleave = strided ? 1 : 0;
+        a.xcd_remap = strided && !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
         a.scale = 1.0;
         l.grid = (ntrans + C - 1) / C;
+        plan.launches.push_back(l);
+    }
+
+    // W_n^m table, m in [0, n) (cached per n inside the plan)
+    const void *power_table(int64_t n)
+    {
+        auto it = plan.pow_cache.find(n);
+        if (it != plan.pow_cache.end()) return it->second;
+        void *buf = alloc(n * (int64_t)esz());
+        if (!buf) return nullptr;
+        check(gen_power_table(plan.dtype, buf, n, 1, n, s));
+        plan.pow_cache[n] = buf;
+        return buf;
+    }
+
+    // Persistent row kernel (compiled layout for this n).
+    void add_rows(int64_t n, int64_t nrows, Slot src, Slot dst, int variant)
+    {
+        const void *tw = power_table(n);
+        if (status) return;
+        Launch l{};
+        l.kind = Launch::kRows;
+        l.src = src;
+        l.dst = dst;
+        l.row_n = (int)n;
+        l.row_variant = variant;
+        l.ra.tw = tw;
+        l.ra.nrows = nrows;
+        l.ra.scale = 1.0;
+        const int per_tile = row_rows_per_tile(plan.dtype, (int)n, variant);
+        const int64_t tiles = (nrows + per_tile - 1) / per_tile;
+        const int64_t resident = (int64_t)dev.cus * row_occupancy(plan.dtype, (int)n, variant);
+        l.grid = std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
+        l.block = row_block(plan.dtype, (int)n, variant);
+        l.lds = row_lds_bytes(plan.dtype, (int)n, variant);
         plan.launches.push_back(l);
     }
 
@@ -330,6 +373,13 @@ struct Builder {
         a.st2 = n * s_;
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
+        if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS")) {
+            int variant = 0;
+            if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
+            if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
+            add_rows(n, ntrans, src, dst, variant);
+            return;
+        }
         if (n <= limit && is_5smooth(n)) {
             add_stockham(n, ntrans, a, a, strided, src, dst);
             return;
@@ -477,6 +527,12 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
     }
     const int conj = direction > 0 ? 1 : 0;
     const size_t n = plan.launches.size();
+    // streaming accesses for the plan's input (read once) and its final output (written once);
+    // intermediates keep the default policy so the next pass finds them in L2 / Infinity Cache
+    static const bool nt_enabled = [] {
+        const char *v = std::getenv("AFFT_NT");
+        return !(v && v[0] == '0');
+    }();
     int st = AFFT_SUCCESS;
     for (size_t i = 0; i < n && st == AFFT_SUCCESS; ++i) {
         const Launch &l = plan.launches[i];
@@ -486,14 +542,28 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             FftArgs a = l.fa;
             a.in = slot_ptr(l.src, src_in, d_out, plan.work);
             a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.nt_in = nt_enabled && l.src == Slot::kIn;
+            a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
             e = launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
+        } else if (l.kind == Launch::kRows) {
+            RowArgs a = l.ra;
+            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.nt_in = nt_enabled && l.src == Slot::kIn;
+            a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
+            a.conj_in = first ? conj : 0;
+            a.conj_out = last ? conj : 0;
+            a.scale = last ? scale : 1.0;
+            e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
         } else {
             BluesteinArgs a = l.ba;
             a.in = slot_ptr(l.src, src_in, d_out, plan.work);
             a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.nt_in = nt_enabled && l.src == Slot::kIn;
+            a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;

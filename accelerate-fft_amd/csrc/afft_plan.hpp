@@ -28,10 +28,12 @@ struct DeviceInfo {
 enum class Slot { kIn, kOut, kWork };
 
 struct Launch {
-    enum Kind { kStockham, kBluestein } kind;
+    enum Kind { kStockham, kBluestein, kRows } kind;
     Slot src, dst;
     FftArgs fa;
     BluesteinArgs ba;
+    RowArgs ra;
+    int row_n = 0, row_variant = 0;
     int block;
     int64_t grid;
     size_t lds;
@@ -45,6 +47,7 @@ struct Plan {
     std::vector<void *> allocs;
     int64_t device_bytes = 0;
     std::map<int64_t, const void *> tw_cache;
+    std::map<int64_t, const void *> pow_cache;
     std::map<int64_t, std::pair<const void *, const void *>> blue_cache;
     int64_t work_elems = 0;
     void *work = nullptr;

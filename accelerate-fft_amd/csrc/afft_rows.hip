@@ -1,0 +1,357 @@
+// afft_rows.hip -- persistent, software-pipelined kernels for batches of contiguous rows of the
+// common lengths (the `fft` innermost-axis hot path, FFT.hs:63-84).
+//
+// Kernel names: afft_rows<Real, N, E, PF, W, R0, R1, ...>  (N = length, E = elements per thread,
+// PF = prefetch the next row, W = waves per SIMD the register allocation must allow, R = pass
+// radices).
+//
+// Why a second kernel family next to afft_stockham: the batched transform is HBM-bound, and the
+// generic kernel leaves the memory system idle while a workgroup computes (LDS caps a CU at a
+// few resident rows).  Here every workgroup loops over row tiles and issues the loads of tile
+// g + gridDim.x before computing tile g, so each resident workgroup keeps a second row in
+// flight in registers.  Everything the generic kernel computes at run time (pass strides,
+// modulo, addresses) is a compile-time constant here.
+//
+// Ordering constraint of the hardware: vector-memory loads and stores retire in issue order
+// (one vmcnt counter), so a wait for any load also waits for every older load and store.  The
+// row-invariant twiddle bases are therefore loaded once before the row loop; inside the loop
+// the only vector-memory operations are the prefetch and the stores, and the LDS exchanges
+// synchronise with LDS-only fences (s_waitcnt lgkmcnt + s_barrier), which leave the prefetch
+// and the stores in flight.
+//
+// Twiddles: pass p (radix R, stride Ns) needs W_{Ns R}^{r k}, r < R, k = b mod Ns.  Each
+// thread loads W^k and W^{4k} from the FP64-generated table W_N^m (rounded to Real) and forms
+// the others by at most three complex products (W^2k = (W^k)^2, W^3k = W^2k W^k, W^(4a+b)k =
+// W^4ak W^bk ...), i.e. within a few ulp of the correctly rounded value.
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "afft_device.hpp"
+#include "afft_internal.hpp"
+
+namespace afft {
+
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int... Rs> struct Radices {
+    static constexpr int NP = sizeof...(Rs);
+    static constexpr int R[NP] = {Rs...};
+    static constexpr int ns(int p)
+    {
+        int v = 1;
+        for (int i = 0; i < p; ++i) v *= R[i];
+        return v;
+    }
+};
+
+template <int P> struct IC {
+    static constexpr int value = P;
+};
+
+template <typename F, int... Ps>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Ps...>)
+{
+    (f(IC<Ps>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// w[r] = W^{r k} for r = 1..R-1 from b1 = W^k and b4 = W^{4k} (b4 used when R > 4).
+template <int R, typename V> __device__ __forceinline__ void derive_twiddles(V *w, V b1, V b4)
+{
+    w[1] = b1;
+    if constexpr (R > 2) w[2] = c_mul(b1, b1);
+    if constexpr (R > 3) w[3] = c_mul(w[2], b1);
+    if constexpr (R == 5) w[4] = c_mul(w[2], w[2]);
+    if constexpr (R > 5) {
+        w[4] = b4;
+#pragma unroll
+        for (int r = 5; r < R; ++r) {
+            const int q = r & ~3, m = r & 3;
+            if (m == 0) w[r] = c_mul(w[q - 4], b4);
+            else w[r] = c_mul(w[q], w[m]);
+        }
+    }
+}
+
+// v[r] *= W^{r k} for r = 1..R-1, the powers formed on the fly from b1 = W^k, b4 = W^{4k}
+// (the same products as derive_twiddles, with only the bases w1..w3, w4, w8, w12 kept live).
+template <int R, typename V> __device__ __forceinline__ void apply_twiddles(V *v, V b1, V b4)
+{
+    v[1] = c_mul(v[1], b1);
+    if constexpr (R > 2) {
+        const V w2 = c_mul(b1, b1);
+        v[2] = c_mul(v[2], w2);
+        if constexpr (R > 3) {
+            const V w3 = c_mul(w2, b1);
+            v[3] = c_mul(v[3], w3);
+            if constexpr (R == 5) v[4] = c_mul(v[4], c_mul(w2, w2));
+            if constexpr (R > 5) {
+                V w4 = b4;
+#pragma unroll
+                for (int q = 4; q < R; q += 4) {
+                    if (q > 4) w4 = c_mul(w4, b4); // W^{q k}
+                    v[q] = c_mul(v[q], w4);
+                    v[q + 1] = c_mul(v[q + 1], c_mul(w4, b1));
+                    v[q + 2] = c_mul(v[q + 2], c_mul(w4, w2));
+                    v[q + 3] = c_mul(v[q + 3], c_mul(w4, w3));
+                }
+            }
+        }
+    }
+}
+
+template <int N, int E> struct RowShape {
+    static constexpr int T = N / E;
+    static constexpr int C = T >= 256 ? 1 : 256 / T;
+    static constexpr int BLOCK = C * T;
+};
+
+template <typename Real, int N, int E, bool PF, int W, int... Rs>
+__global__ void __launch_bounds__((RowShape<N, E>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W, 8)))
+afft_rows(const RowArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    using RL = Radices<Rs...>;
+    constexpr int T = RowShape<N, E>::T;
+    constexpr int C = RowShape<N, E>::C;
+    constexpr int NP = RL::NP;
+    constexpr int PADN = lds_padded_len(N);
+    static_assert(T * E == N, "E must divide N");
+
+    extern __shared__ __align__(16) unsigned char afft_smem[];
+    V *lds = reinterpret_cast<V *>(afft_smem);
+    const int t = threadIdx.x % T;
+    const int c = threadIdx.x / T;
+    V *slot = lds + c * PADN;
+    const V *__restrict__ in = reinterpret_cast<const V *>(a.in);
+    V *__restrict__ out = reinterpret_cast<V *>(a.out);
+    const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
+    const int64_t nrows = a.nrows;
+    const int64_t ntiles = (nrows + C - 1) / C;
+    const Real sc = (Real)a.scale;
+    const bool cin = a.conj_in, cout = a.conj_out;
+
+    const bool nt_in = a.nt_in, nt_out = a.nt_out;
+    V nxt[E];
+    auto load_tile = [&](int64_t tile) {
+        const int64_t row = tile * C + c;
+        if (tile < ntiles && row < nrows) {
+            const V *p = in + row * N + t;
+            if (nt_in) {
+#pragma unroll
+                for (int i = 0; i < E; ++i) nxt[i] = c_load_nt(p + i * T);
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; ++i) nxt[i] = p[i * T];
+            }
+        }
+    };
+
+    // base twiddles of every pass: they depend on the thread, not on the row, so they are
+    // loaded once, before any row load, and stay in registers
+    V tb[NP > 0 ? NP : 1][2 * E];
+    static_for<NP>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        constexpr int R = RL::R[p];
+        constexpr int Ns = RL::ns(p);
+        if constexpr (Ns > 1) {
+            constexpr int B = E / R;
+            constexpr int NB = (T % Ns == 0) ? 1 : B;
+            constexpr int STEP = N / (Ns * R);
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) {
+                const int k = (t + jb * T) % Ns;
+                tb[p][2 * jb] = tw[k * STEP];
+                tb[p][2 * jb + 1] = (R > 5) ? tw[4 * k * STEP] : tb[p][2 * jb];
+            }
+        }
+    });
+
+    int64_t g = blockIdx.x;
+    if constexpr (PF) load_tile(g);
+    for (; g < ntiles; g += gridDim.x) {
+        if constexpr (!PF) load_tile(g);
+        V reg[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
+        if constexpr (PF) {
+            __builtin_amdgcn_sched_barrier(0);
+            load_tile(g + gridDim.x);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        static_for<NP>([&](auto pc) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int R = RL::R[p];
+            constexpr int Ns = RL::ns(p);
+            constexpr int B = E / R;
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                V v[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = reg[j + r * B];
+                if constexpr (Ns > 1) {
+                    constexpr int NB = (T % Ns == 0) ? 1 : B;
+                    const int jb = NB == 1 ? 0 : j;
+                    apply_twiddles<R>(v, tb[p][2 * jb], tb[p][2 * jb + 1]);
+                }
+                dft<R>::run(v);
+#pragma unroll
+                for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
+            }
+            if constexpr (p + 1 < NP) {
+                lds_sync(); // the previous exchange has been read by everyone
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    const int b = t + j * T;
+                    const int k = b % Ns;
+                    const int dst = (b - k) * R + k;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) slot[lds_pad(dst + r * Ns)] = reg[j + r * B];
+                }
+                lds_sync();
+#pragma unroll
+                for (int i = 0; i < E; ++i) reg[i] = slot[lds_pad(t + i * T)];
+            }
+        });
+
+        const int64_t row = g * C + c;
+        if (row < nrows) {
+            V *q = out + row * N + t;
+            if (nt_out) {
+#pragma unroll
+                for (int i = 0; i < E; ++i) c_store_nt(q + i * T, c_scale(cout ? c_conj(reg[i]) : reg[i], sc));
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; ++i) q[i * T] = c_scale(cout ? c_conj(reg[i]) : reg[i], sc);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Registry of the instantiated (dtype, N) layouts.
+
+namespace {
+
+using RowFn = void (*)(const RowArgs);
+
+struct RowEntry {
+    int dtype;
+    int n;
+    int E;
+    bool pf;
+    RowFn fn;
+    int block;
+};
+
+#define AFFT_ROW(REAL, DT, N, E, PF, W, ...)                                                     \
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, __VA_ARGS__>, RowShape<N, E>::BLOCK}
+
+// The first entry of a (dtype, n) is the default; the others are tuning alternatives.
+const RowEntry kRows[] = {
+    AFFT_ROW(float, 0, 256, 16, true, 3, 16, 16),
+    AFFT_ROW(float, 0, 512, 16, true, 3, 16, 16, 2),
+    AFFT_ROW(float, 0, 1024, 16, true, 3, 16, 16, 4),
+    AFFT_ROW(float, 0, 2048, 16, true, 3, 16, 16, 8),
+    AFFT_ROW(float, 0, 4096, 16, true, 2, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 16, true, 3, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 16, false, 3, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 16, false, 4, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 32, true, 2, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 32, false, 2, 16, 16, 16),
+    AFFT_ROW(float, 0, 8192, 16, true, 3, 16, 16, 16, 2),
+    AFFT_ROW(float, 0, 16384, 16, false, 4, 16, 16, 16, 4),
+    AFFT_ROW(double, 1, 256, 16, true, 2, 16, 16),
+    AFFT_ROW(double, 1, 512, 16, true, 2, 16, 16, 2),
+    AFFT_ROW(double, 1, 1024, 16, true, 2, 16, 16, 4),
+    AFFT_ROW(double, 1, 2048, 16, true, 2, 16, 16, 8),
+    AFFT_ROW(double, 1, 4096, 16, true, 2, 16, 16, 16),
+    AFFT_ROW(double, 1, 4096, 16, false, 2, 16, 16, 16),
+};
+
+const RowEntry *find_row(int dtype, int n, int variant)
+{
+    const RowEntry *first = nullptr;
+    int seen = 0;
+    for (const RowEntry &e : kRows)
+        if (e.dtype == dtype && e.n == n) {
+            if (!first) first = &e;
+            if (seen++ == variant) return &e;
+        }
+    return variant == 0 ? first : nullptr;
+}
+
+} // namespace
+
+int row_variants(int dtype, int n)
+{
+    int v = 0;
+    for (const RowEntry &e : kRows)
+        if (e.dtype == dtype && e.n == n) ++v;
+    return v;
+}
+
+int row_block(int dtype, int n, int variant)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    return e ? e->block : 0;
+}
+
+size_t row_lds_bytes(int dtype, int n, int variant)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    if (!e) return 0;
+    const int T = e->n / e->E;
+    const int C = T >= 256 ? 1 : 256 / T;
+    return (size_t)C * lds_padded_len(e->n) * (dtype == 0 ? 8 : 16);
+}
+
+int row_rows_per_tile(int dtype, int n, int variant)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    if (!e) return 0;
+    const int T = e->n / e->E;
+    return T >= 256 ? 1 : 256 / T;
+}
+
+int row_occupancy(int dtype, int n, int variant)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    if (!e) return 0;
+    const size_t lds = row_lds_bytes(dtype, n, variant);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(e->fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void *>(e->fn),
+                                                     e->block, lds) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+hipError_t launch_rows(int dtype, int n, int variant, const RowArgs &a, int64_t grid, hipStream_t s)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    if (!e) return hipErrorInvalidValue;
+    if (grid <= 0) return hipSuccess;
+    const size_t lds = row_lds_bytes(dtype, n, variant);
+    if (lds > 64 * 1024) {
+        hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void *>(e->fn),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(e->fn, dim3((unsigned)grid), dim3(e->block), lds, s, a);
+    return hipGetLastError();
+}
+
+} // namespace afft

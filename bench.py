@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Throughput of the FFT engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu] [--no-extras]
+
+Workload (the headline line): `fft` Forward along the innermost axis of an Array DIM2
+(Complex Float) of 16384 x 4096, re/im uniform[-1, 1) -- 16384 independent 4096-point
+transforms, inputs resident in HBM before the timed region.  One step = one call.
+
+  value      GFLOP/s by the 5 N log2 N convention: 16384 * 5 * 4096 * 12 flop per step.
+  roofline   HBM-bound; achieved = algorithmic bytes (one read + one write of the array,
+             2 * 16384 * 4096 * 8 B = 1 GiB per launch) / average kernel duration, timed with
+             HIP events on the stream the kernel is launched on; peak 8 TB/s.  `traffic` is the
+             HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a
+             copy kernel of known size), read from profiles/ when present.
+  cpu_baseline  the oracle's recursive FFT in float arithmetic (the CPU path restated; FFTW is
+             not installed), over the same 16384 x 4096 array, on every host core available.
+  extras     the other configurations (four-step 2^22 row, Reverse/Inverse, fft2D 4096^2,
+             fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
+             time and roofline fraction.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "accelerate-fft_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FFT GFLOP/s (5N log2N) + % HBM roofline; fwd+adjoint time vs plain, same GPU"
+
+
+def uniform_c(shape, dtype, seed, device):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    real = torch.float32 if dtype == torch.complex64 else torch.float64
+    re = torch.rand(shape, generator=g, device=device, dtype=real) * 2 - 1
+    im = torch.rand(shape, generator=g, device=device, dtype=real) * 2 - 1
+    return torch.complex(re, im)
+
+
+def normal_c(shape, dtype, seed, device):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    real = torch.float32 if dtype == torch.complex64 else torch.float64
+    return torch.complex(torch.randn(shape, generator=g, device=device, dtype=real),
+                         torch.randn(shape, generator=g, device=device, dtype=real))
+
+
+def time_calls(fn, steps, warmup):
+    """Average seconds per call, HIP events on the current stream (the one the library
+    launches on) bracketing `steps` calls after `warmup` untimed ones."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(steps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / 1e3 / steps
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(rows, n, min_seconds=8.0, max_reps=6):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    x = O.uniform(2, (rows, n), np.complex64)
+    O.fft_rows_f32(x[:64], -1, nthreads=cores)  # warm-up, twiddles
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.fft_rows_f32(x, -1, nthreads=cores)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or reps >= max_reps:
+            break
+    flops = 5.0 * n * math.log2(n) * rows * reps
+    return {"value": round(flops / el / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"oracle recursive radix-4/2 FFT in float, fft Forward over the full {rows}x{n} "
+                      f"Complex Float array, {reps} reps in {el:.1f} s on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import afft
+    from afft import Mode
+
+    dev = torch.device("cuda", 0)
+    rows, n = 16384, 4096
+    x = uniform_c((rows, n), torch.complex64, 2, dev)
+    out = torch.empty_like(x)
+    plan = afft.Plan((rows, n), torch.complex64, "innermost")
+    nl = plan.info()["launches"]
+
+    def step():
+        plan.execute(x, -1, 1.0, out=out)
+
+    # headline: K timed steps between barriers (synchronize) on both sides
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    kern = time_calls(step, args.steps, 2)  # event-timed on the launch stream
+    flops = 5.0 * n * math.log2(n) * rows
+    alg_bytes = 2.0 * rows * n * 8
+    achieved = alg_bytes / kern / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(flops / wall / 1e9, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: re/im uniform[-1,1) from torch.rand seed 2, resident in HBM",
+        "config": {"workload": "fft Forward along the innermost axis of Array DIM2 (Complex Float) 16384x4096",
+                   "rows": rows, "n": n, "mode": "Forward", "launches_per_step": nl},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("afft_stockham_f32_e16_fft_16384x4096_traffic.txt"),
+                     "kernel_avg_us": round(kern * 1e6, 2)},
+    }
+    del plan
+
+    extras = {}
+    if not args.no_extras:
+        def add(name, fn, n_total, bytes_alg, flops_call):
+            try:
+                t = time_calls(fn, max(5, args.steps // 2), 3)
+                extras[name] = {"us": round(t * 1e6, 2), "gflops": round(flops_call / t / 1e9, 1),
+                                "hbm_frac": round(bytes_alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+            except afft.AfftError as e:
+                extras[name] = {"unavailable": str(e)}
+
+        for mode in (Mode.Reverse, Mode.Inverse):
+            add(f"fft_16384x4096_c64_{mode.name}", lambda m=mode: afft.fft(m, x), rows * n, alg_bytes, flops)
+        del x, out
+        torch.cuda.empty_cache()
+        big = uniform_c((1, 1 << 22), torch.complex64, 2, dev)
+        N = 1 << 22
+        add("fft_1x4194304_c64_four_step", lambda: afft.fft(Mode.Forward, big), N, 16.0 * N, 5.0 * N * 22)
+        del big
+        m2 = normal_c((4096, 4096), torch.complex64, 3, dev)
+        N = 4096 * 4096
+        add("fft2D_4096x4096_c64_Forward", lambda: afft.fft2D(Mode.Forward, m2), N, 16.0 * N, 5.0 * N * 24)
+        add("fft2D_4096x4096_c64_Inverse", lambda: afft.fft2D(Mode.Inverse, m2), N, 16.0 * N, 5.0 * N * 24)
+        del m2
+        c3 = normal_c((256, 256, 256), torch.complex128, 3, dev)
+        N = 256 ** 3
+        add("fft3D_256x256x256_c128_Forward", lambda: afft.fft3D(Mode.Forward, c3), N, 32.0 * N, 5.0 * N * 24)
+        del c3
+        torch.cuda.empty_cache()
+        line["extras"] = extras
+
+    if not args.no_cpu:
+        cb = cpu_baseline(rows, n)
+        line["cpu_baseline"] = cb
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
