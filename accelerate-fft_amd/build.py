@@ -38,6 +38,9 @@ def _compile(src):
 
 
 def build(jobs=None, verbose=True):
+    stems = [os.path.splitext(f)[0] for f in SOURCES]
+    if len(set(stems)) != len(stems):
+        raise RuntimeError("two sources share an object name: " + ", ".join(SOURCES))
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     jobs = jobs or min(8, len(SOURCES))

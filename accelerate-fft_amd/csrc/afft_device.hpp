@@ -333,9 +333,11 @@ __device__ __forceinline__ void scatter_dispatch(int R, const V (&reg)[E], V *ld
 // out).  All threads of the workgroup must call it (it contains barriers); threads that own no
 // transform (tail of the batch) still take part.  `lds_base`/`es` address this thread's
 // transform in LDS; `tw` holds the pass twiddles (pass p at tw + pt.tw_off[p]).
+// `on` = false for threads of the workgroup that own no transform slot: they take part in the
+// barriers but never touch LDS.
 template <int E, typename V>
 __device__ __forceinline__ void fft_core(V (&reg)[E], V *lds_base, int es, const PassTable &pt,
-                                         const V *__restrict__ tw, int t, int T)
+                                         const V *__restrict__ tw, int t, int T, bool on = true)
 {
     int Ns = 1;
     for (int p = 0; p < pt.npass; ++p) {
@@ -343,9 +345,9 @@ __device__ __forceinline__ void fft_core(V (&reg)[E], V *lds_base, int es, const
         pass_dispatch<E>(R, reg, tw + pt.tw_off[p], Ns, t, T);
         if (p + 1 < pt.npass) {
             __syncthreads(); // everyone is done reading the previous exchange
-            scatter_dispatch<E>(R, reg, lds_base, es, Ns, t, T);
+            if (on) scatter_dispatch<E>(R, reg, lds_base, es, Ns, t, T);
             __syncthreads();
-            natural_read<E>(reg, lds_base, es, t, T);
+            if (on) natural_read<E>(reg, lds_base, es, t, T);
         }
         Ns *= R;
     }

@@ -64,6 +64,24 @@ struct BluesteinArgs {
     int nt_in, nt_out;
 };
 
+// Pointwise stages of the multi-kernel Bluestein path (lengths whose padded size M exceeds one
+// workgroup).  mode 0 (pre):  aux[j*M + m] = x_j[m] c[m] for m < n, 0 for n <= m < M
+//                mode 1 (mid):  aux[j*M + m] = conj(aux[j*M + m] * filt[m])
+//                mode 2 (post): y_j[m] = conj(aux[j*M + m]) c[m] for m < n
+struct ChirpArgs {
+    const void *in;
+    void *out;
+    const void *chirp;
+    const void *filt;
+    int64_t ntrans, n, M;
+    Addr a;              // addressing of the caller's array (x for pre, y for post)
+    int mode;
+    int conj_in, conj_out;
+    double scale;
+    int nt_in, nt_out;
+};
+hipError_t launch_chirp(int dtype, const ChirpArgs &a, hipStream_t s);
+
 // Persistent batch of contiguous rows of a length with a compiled layout (afft_rows.hip).
 struct RowArgs {
     const void *in;

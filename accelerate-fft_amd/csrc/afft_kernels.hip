@@ -185,6 +185,44 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 }
 
 // ----------------------------------------------------------------------------------------------
+// Pointwise stages of the multi-kernel Bluestein path.
+
+template <typename Real>
+__global__ void __launch_bounds__(256) afft_chirp(const ChirpArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    const V *chirp = reinterpret_cast<const V *>(a.chirp);
+    const int64_t per = a.mode == 2 ? a.n : a.M;
+    const int64_t total = a.ntrans * per;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx / per;
+        const int64_t m = idx - j * per;
+        if (a.mode == 0) {
+            const V *x = reinterpret_cast<const V *>(a.in);
+            V *aux = reinterpret_cast<V *>(a.out);
+            V v = c_make<V>(0, 0);
+            if (m < a.n) {
+                v = c_load(x + addr_base(a.a, j) + m * a.a.es, (bool)a.nt_in);
+                if (a.conj_in) v = c_conj(v);
+                v = c_mul(v, chirp[m]);
+            }
+            aux[j * a.M + m] = v;
+        } else if (a.mode == 1) {
+            V *aux = reinterpret_cast<V *>(a.out);
+            const V *filt = reinterpret_cast<const V *>(a.filt);
+            aux[j * a.M + m] = c_conj(c_mul(aux[j * a.M + m], filt[m]));
+        } else {
+            const V *aux = reinterpret_cast<const V *>(a.in);
+            V *y = reinterpret_cast<V *>(a.out);
+            V v = c_mul(c_conj(aux[j * a.M + m]), chirp[m]);
+            if (a.conj_out) v = c_conj(v);
+            c_store(y + addr_base(a.a, j) + m * a.a.es, c_scale(v, (Real)a.scale), (bool)a.nt_out);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
 // Twiddle / chirp generation in FP64.
 
 template <typename Real>
@@ -394,6 +432,19 @@ hipError_t convert_scale_from_f64(int dtype, void *out, const void *in, int64_t 
     else
         hipLaunchKernelGGL(afft_convert_scale<double>, dim3(g), dim3(256), 0, s, (double2 *)out,
                            (const double2 *)in, count, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_chirp(int dtype, const ChirpArgs &a, hipStream_t s)
+{
+    const int64_t per = a.mode == 2 ? a.n : a.M;
+    int64_t g = (a.ntrans * per + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) return hipSuccess;
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_chirp<float>, dim3((unsigned)g), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(afft_chirp<double>, dim3((unsigned)g), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -2,15 +2,6 @@
 // returns AFFT_ERROR_INTERNAL so a caller gets a clean status instead of a missing symbol.
 #include "../../include/afft.h"
 
-extern "C" int afft_ragged_plan_create(afft_ragged_plan **plan, int, int64_t, const int64_t *, int)
-{
-    if (plan) *plan = nullptr;
-    return AFFT_ERROR_INTERNAL;
-}
-extern "C" int afft_ragged_exec(const afft_ragged_plan *, int, const void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
-extern "C" int afft_ragged_exec_graph(afft_ragged_plan *, int, const void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
-extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *) { return AFFT_ERROR_INTERNAL; }
-extern "C" int afft_ragged_total(const afft_ragged_plan *, int64_t *) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_spectral_conv_fwd(int, int64_t, int64_t, int64_t, const void *, const void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_spectral_conv_vjp(int, int64_t, int64_t, int64_t, const void *, const void *, const void *, void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_autotune(int, int, const int64_t *, int, int) { return AFFT_ERROR_INTERNAL; }

@@ -38,7 +38,7 @@ static bool getenv_flag(const char *name)
     return v && v[0] && v[0] != '0';
 }
 
-static bool is_5smooth(int64_t n)
+bool is_5smooth(int64_t n)
 {
     if (n < 1) return false;
     for (int p : {2, 3, 5})
@@ -65,7 +65,7 @@ int64_t lds_limit_elems(int dtype, size_t lds_bytes)
     return n;
 }
 
-bool choose_layout(int64_t n, int dtype, LdsLayout &out)
+bool choose_layout(int64_t n, int dtype, LdsLayout &out, int maxE, int maxT)
 {
     if (n < 1 || n > (1 << 30)) return false;
     if (n == 1) {
@@ -85,6 +85,7 @@ bool choose_layout(int64_t n, int dtype, LdsLayout &out)
         if ((prime_mask(E) & need) != need) continue;
         const int64_t T = n / E;
         if (T > max_block_threads(dtype == AFFT_C128, E)) continue;
+        if (E > maxE || (maxT > 0 && T > maxT)) continue;
         if (dtype == AFFT_C128 && E > 32) continue; // FP64 data beyond 128 VGPRs
         LdsLayout L{};
         L.E = E;
@@ -254,50 +255,93 @@ struct Builder {
         plan.launches.push_back(l);
     }
 
+    // Chirp c[m] (m < n) and filter FFT_M(b)/M for a Bluestein length, cached per n.  The
+    // filter is transformed in FP64 by a sub-plan of this engine and then rounded.
+    bool bluestein_tables(int64_t n, int64_t M, const void *&chirp, const void *&filt)
+    {
+        auto it = plan.blue_cache.find(n);
+        if (it != plan.blue_cache.end()) {
+            chirp = it->second.first;
+            filt = it->second.second;
+            return true;
+        }
+        void *cbuf = alloc(n * (int64_t)esz());
+        void *fbuf = alloc(M * (int64_t)esz());
+        if (status) return false;
+        check(gen_chirp(plan.dtype, cbuf, n, s));
+        void *b64 = nullptr;
+        if (hipMalloc(&b64, (size_t)M * 16) != hipSuccess) {
+            status = AFFT_ERROR_ALLOC;
+            return false;
+        }
+        check(gen_bluestein_kernel_f64(b64, n, M, s));
+        Plan sub;
+        int64_t ext[1] = {M};
+        int st = build_plan(sub, AFFT_C128, 1, ext, AFFT_AXES_INNERMOST, plan.device, dev, s);
+        if (st == AFFT_SUCCESS) st = exec_plan(sub, -1, b64, b64, 1.0, s);
+        if (st != AFFT_SUCCESS && status == AFFT_SUCCESS) status = st;
+        check(convert_scale_from_f64(plan.dtype, fbuf, b64, M, 1.0 / (double)M, s));
+        check(hipStreamSynchronize(s));
+        (void)hipFree(b64);
+        free_plan(sub);
+        if (status) return false;
+        plan.blue_cache[n] = {cbuf, fbuf};
+        chirp = cbuf;
+        filt = fbuf;
+        return true;
+    }
+
+    // Bluestein over several launches when M exceeds one workgroup: pre-multiply into the
+    // auxiliary buffer, FFT_M, filter, FFT_M, post-multiply.
+    void add_bluestein_large(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, Slot src,
+                             Slot dst)
+    {
+        const int64_t M = bluestein_length(n);
+        const void *chirp = nullptr, *filt = nullptr;
+        if (!bluestein_tables(n, M, chirp, filt)) return;
+        plan.aux_elems = std::max(plan.aux_elems, ntrans * M);
+        Launch l{};
+        l.kind = Launch::kChirp;
+        l.ca.chirp = chirp;
+        l.ca.filt = filt;
+        l.ca.ntrans = ntrans;
+        l.ca.n = n;
+        l.ca.M = M;
+        l.ca.scale = 1.0;
+        Launch pre = l, mid = l, post = l;
+        pre.src = src;
+        pre.dst = Slot::kAux;
+        pre.ca.a = ain;
+        pre.ca.mode = 0;
+        mid.src = mid.dst = Slot::kAux;
+        mid.ca.mode = 1;
+        post.src = Slot::kAux;
+        post.dst = dst;
+        post.ca.a = aout;
+        post.ca.mode = 2;
+        plan.launches.push_back(pre);
+        add_axis(M, 1, ntrans, Slot::kAux, Slot::kAux);
+        if (status) return;
+        plan.launches.push_back(mid);
+        add_axis(M, 1, ntrans, Slot::kAux, Slot::kAux);
+        if (status) return;
+        plan.launches.push_back(post);
+    }
+
     void add_bluestein(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, bool strided,
                        Slot src, Slot dst)
     {
         const int64_t M = bluestein_length(n);
         LdsLayout L{};
         if (M > dev.max_lds_len[plan.dtype] || !choose_layout(M, plan.dtype, L)) {
-            status = AFFT_ERROR_UNSUPPORTED_SIZE;
+            add_bluestein_large(n, ntrans, ain, aout, src, dst);
             return;
         }
         PassTable pt{};
         const void *tw = twiddles(M, L, pt);
         if (status) return;
-        // chirp and filter, cached per n
-        auto it = plan.blue_cache.find(n);
         const void *chirp = nullptr, *filt = nullptr;
-        if (it != plan.blue_cache.end()) {
-            chirp = it->second.first;
-            filt = it->second.second;
-        } else {
-            void *cbuf = alloc(n * (int64_t)esz());
-            void *fbuf = alloc(M * (int64_t)esz());
-            if (status) return;
-            check(gen_chirp(plan.dtype, cbuf, n, s));
-            // b in FP64, its FP64 FFT by a sub-plan of this engine, scaled by 1/M and rounded
-            void *b64 = nullptr;
-            if (hipMalloc(&b64, (size_t)M * 16) != hipSuccess) {
-                status = AFFT_ERROR_ALLOC;
-                return;
-            }
-            check(gen_bluestein_kernel_f64(b64, n, M, s));
-            Plan sub;
-            int64_t ext[1] = {M};
-            int st = build_plan(sub, AFFT_C128, 1, ext, AFFT_AXES_INNERMOST, plan.device, dev, s);
-            if (st == AFFT_SUCCESS) st = exec_plan(sub, -1, b64, b64, 1.0, s);
-            if (st != AFFT_SUCCESS) status = st;
-            check(convert_scale_from_f64(plan.dtype, fbuf, b64, M, 1.0 / (double)M, s));
-            check(hipStreamSynchronize(s));
-            (void)hipFree(b64);
-            free_plan(sub);
-            if (status) return;
-            plan.blue_cache[n] = {cbuf, fbuf};
-            chirp = cbuf;
-            filt = fbuf;
-        }
+        if (!bluestein_tables(n, M, chirp, filt)) return;
         Launch l{};
         l.kind = Launch::kBluestein;
         l.src = src;
@@ -484,6 +528,9 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     if (b.status == AFFT_SUCCESS && plan.work_elems > 0) {
         plan.work = b.alloc(plan.work_elems * (int64_t)b.esz());
     }
+    if (b.status == AFFT_SUCCESS && plan.aux_elems > 0) {
+        plan.aux = b.alloc(plan.aux_elems * (int64_t)b.esz());
+    }
     if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
     // some four-step passes transpose: in-place execution needs a private copy of the input
     for (const Launch &l : plan.launches)
@@ -500,12 +547,13 @@ void free_plan(Plan &plan)
     plan.blue_cache.clear();
 }
 
-static void *slot_ptr(Slot s, const void *in, void *out, void *work)
+static void *slot_ptr(Slot s, const void *in, void *out, const Plan &plan)
 {
     switch (s) {
     case Slot::kIn: return const_cast<void *>(in);
     case Slot::kOut: return out;
-    case Slot::kWork: return work;
+    case Slot::kWork: return plan.work;
+    case Slot::kAux: return plan.aux;
     }
     return nullptr;
 }
@@ -540,8 +588,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
         hipError_t e;
         if (l.kind == Launch::kStockham) {
             FftArgs a = l.fa;
-            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.in = slot_ptr(l.src, src_in, d_out, plan);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -550,18 +598,28 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             e = launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
         } else if (l.kind == Launch::kRows) {
             RowArgs a = l.ra;
-            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.in = slot_ptr(l.src, src_in, d_out, plan);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
             e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
+        } else if (l.kind == Launch::kChirp) {
+            ChirpArgs a = l.ca;
+            a.in = slot_ptr(l.src, src_in, d_out, plan);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.nt_in = nt_enabled && l.src == Slot::kIn;
+            a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
+            a.conj_in = first ? conj : 0;
+            a.conj_out = last ? conj : 0;
+            a.scale = last ? scale : 1.0;
+            e = launch_chirp(plan.dtype, a, s);
         } else {
             BluesteinArgs a = l.ba;
-            a.in = slot_ptr(l.src, src_in, d_out, plan.work);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan.work);
+            a.in = slot_ptr(l.src, src_in, d_out, plan);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -695,6 +753,42 @@ int current_device(int &dev)
 {
     if (hipGetDevice(&dev) != hipSuccess) return AFFT_ERROR_NO_DEVICE;
     return AFFT_SUCCESS;
+}
+
+// Table helpers for the engines outside the axis planner (ragged batches, spectral
+// convolution): the allocations belong to `owner` and are released by free_plan(owner).
+void *plan_alloc(Plan &owner, const DeviceInfo &dev, int64_t bytes, int &status)
+{
+    Builder b{owner, dev, nullptr};
+    void *p = b.alloc(bytes);
+    if (b.status) status = b.status;
+    return p;
+}
+
+int table_pass_twiddles(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n,
+                        const LdsLayout &L, PassTable &pt, const void *&tw)
+{
+    Builder b{owner, dev, s};
+    tw = b.twiddles(n, L, pt);
+    return b.status;
+}
+
+int table_power(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t L, int64_t step,
+                int64_t count, const void *&out)
+{
+    Builder b{owner, dev, s};
+    void *buf = b.alloc(count * (int64_t)b.esz());
+    if (!b.status) b.check(gen_power_table(owner.dtype, buf, L, step, count, s));
+    out = buf;
+    return b.status;
+}
+
+int table_bluestein(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n, int64_t M,
+                    const void *&chirp, const void *&filt)
+{
+    Builder b{owner, dev, s};
+    b.bluestein_tables(n, M, chirp, filt);
+    return b.status;
 }
 
 } // namespace afft

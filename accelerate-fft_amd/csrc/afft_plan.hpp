@@ -25,14 +25,15 @@ struct DeviceInfo {
     int64_t max_lds_len[2];  // longest single-workgroup transform per dtype
 };
 
-enum class Slot { kIn, kOut, kWork };
+enum class Slot { kIn, kOut, kWork, kAux };
 
 struct Launch {
-    enum Kind { kStockham, kBluestein, kRows } kind;
+    enum Kind { kStockham, kBluestein, kRows, kChirp } kind;
     Slot src, dst;
     FftArgs fa;
     BluesteinArgs ba;
     RowArgs ra;
+    ChirpArgs ca;
     int row_n = 0, row_variant = 0;
     int block;
     int64_t grid;
@@ -51,10 +52,14 @@ struct Plan {
     std::map<int64_t, std::pair<const void *, const void *>> blue_cache;
     int64_t work_elems = 0;
     void *work = nullptr;
+    int64_t aux_elems = 0;
+    void *aux = nullptr;
     bool inplace_needs_copy = false;
 };
 
-bool choose_layout(int64_t n, int dtype, LdsLayout &out);
+// Single-workgroup layout of length n: fewest passes, then E closest to 16; optional limits on
+// elements per thread and threads per transform.
+bool choose_layout(int64_t n, int dtype, LdsLayout &out, int maxE = 1 << 30, int maxT = -1);
 int64_t bluestein_length(int64_t n);
 int64_t lds_limit_elems(int dtype, size_t lds_bytes);
 int device_info(int device, DeviceInfo &out);
@@ -68,5 +73,13 @@ int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, i
 int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream);
 int current_device(int &dev);
 void clear_cache();
+void *plan_alloc(Plan &owner, const DeviceInfo &dev, int64_t bytes, int &status);
+int table_pass_twiddles(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n,
+                        const LdsLayout &L, PassTable &pt, const void *&tw);
+int table_power(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t L, int64_t step,
+                int64_t count, const void *&out);
+int table_bluestein(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n, int64_t M,
+                    const void *&chirp, const void *&filt);
+bool is_5smooth(int64_t n);
 
 } // namespace afft

@@ -128,7 +128,8 @@ def test_long_rows_four_step(gpu, dtype, n):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_bluestein_primes_and_ragged_lengths(gpu, dtype):
-    for n in (257, 1009, 4099, 3 * 7 * 11, 2 * 1021):
+    # 4099 (Complex Double) and 10007 pad beyond one workgroup: the multi-kernel path
+    for n in (257, 1009, 4099, 3 * 7 * 11, 2 * 1021, 10007):
         x = O.uniform(n, (3, n))
         xd = dev(x, dtype, gpu)
         xr = host(xd)
@@ -213,8 +214,11 @@ def test_errors_are_statuses(gpu):
         afft.fft(Mode.Forward, torch.zeros(8, device=gpu))
     with pytest.raises(ValueError):
         afft.fft2D(Mode.Forward, torch.zeros(8, dtype=torch.complex64, device=gpu))
-    # a length that no single workgroup, four-step or Bluestein path supports
-    big_prime = 1000003
-    with pytest.raises(afft.AfftError) as e:
-        afft.fft(Mode.Forward, torch.zeros(1, big_prime, dtype=torch.complex64, device=gpu))
-    assert e.value.status == 4
+    # a 5-smooth length beyond two single-workgroup factors (16384^2 for Complex Float) is
+    # refused at planning time, before any allocation or launch
+    lib = afft._lib.lib()
+    buf = torch.zeros(4, dtype=torch.complex64, device=gpu)
+    ext = afft._lib.extents_arg([3 << 29])
+    st = lib.afft_fft(0, 0, 1, ext, ctypes.c_void_p(buf.data_ptr()), ctypes.c_void_p(buf.data_ptr()),
+                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert st == 4
