@@ -27,7 +27,7 @@ import torch
 from . import _lib as L
 from ._lib import AfftError
 
-__all__ = ["Mode", "fft", "fft1D", "fft2D", "fft3D", "vjp", "Plan", "AfftError",
+__all__ = ["Mode", "fft", "fft1D", "fft2D", "fft3D", "fft2D_many", "vjp", "Plan", "AfftError",
            "RaggedPlan", "spectral_conv", "spectral_conv_vjp", "describe"]
 
 
@@ -113,6 +113,20 @@ def fft3D(mode, arr):
     with torch.cuda.device(x.device):
         L.check(L.lib().afft_fft3D(_mode(mode), _dtype_code(x), x.shape[0], x.shape[1],
                                    x.shape[2], _ptr(x), _ptr(out), _stream(x.device)), "afft_fft3D")
+    return out
+
+
+def fft2D_many(mode, arr):
+    """fft2D of every h x w matrix of a (batch, h, w) array in one call (extension: the
+    reference has no batched 2D entry point, FFT.hs:119-122); Inverse divides by h*w."""
+    x = _device_tensor(arr)
+    if x.dim() != 3:
+        raise ValueError("fft2D_many takes a (batch, h, w) array")
+    out = torch.empty_like(x)
+    with torch.cuda.device(x.device):
+        L.check(L.lib().afft_fft2D_many(_mode(mode), _dtype_code(x), x.shape[0], x.shape[1],
+                                        x.shape[2], _ptr(x), _ptr(out), _stream(x.device)),
+                "afft_fft2D_many")
     return out
 
 

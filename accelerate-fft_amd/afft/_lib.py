@@ -49,6 +49,7 @@ def lib():
     _sig(L.afft_fft1D, i, i, i, i64, vp, vp, vp)
     _sig(L.afft_fft2D, i, i, i, i64, i64, vp, vp, vp)
     _sig(L.afft_fft3D, i, i, i, i64, i64, i64, vp, vp, vp)
+    _sig(L.afft_fft2D_many, i, i, i, i64, i64, i64, vp, vp, vp)
     _sig(L.afft_plan_create, i, pvp, i, i, pi64, i, i)
     _sig(L.afft_plan_cached, i, pvp, i, i, pi64, i, i)
     _sig(L.afft_exec, i, vp, i, vp, vp, d, vp)

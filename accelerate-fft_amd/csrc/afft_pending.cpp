@@ -2,8 +2,6 @@
 // returns AFFT_ERROR_INTERNAL so a caller gets a clean status instead of a missing symbol.
 #include "../../include/afft.h"
 
-extern "C" int afft_spectral_conv_fwd(int, int64_t, int64_t, int64_t, const void *, const void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
-extern "C" int afft_spectral_conv_vjp(int, int64_t, int64_t, int64_t, const void *, const void *, const void *, void *, void *, void *) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_autotune(int, int, const int64_t *, int, int) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_tune_save(const char *) { return AFFT_ERROR_INTERNAL; }
 extern "C" int afft_tune_load(const char *) { return AFFT_ERROR_INTERNAL; }

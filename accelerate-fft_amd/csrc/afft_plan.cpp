@@ -505,7 +505,8 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     Slot src = Slot::kIn;
     int64_t inner = 1;
     for (int ax = rank - 1; ax >= 0; --ax) {
-        const bool take = (axes == AFFT_AXES_ALL) || ax == rank - 1;
+        const bool take = (axes == AFFT_AXES_ALL) || ax == rank - 1 ||
+                          (axes == AFFT_AXES_LAST2 && ax >= rank - 2);
         const int64_t n = extents[ax];
         if (take && n > 1) {
             const int64_t o = plan.total / (n * inner);
@@ -883,6 +884,13 @@ extern "C" int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, 
     return front_end(mode, dtype, 3, e, AFFT_AXES_ALL, d * h * w, d_in, d_out, stream);
 }
 
+extern "C" int afft_fft2D_many(int mode, int dtype, int64_t batch, int64_t h, int64_t w,
+                               const void *d_in, void *d_out, void *stream)
+{
+    const int64_t e[3] = {batch, h, w};
+    return front_end(mode, dtype, 3, e, AFFT_AXES_LAST2, h * w, d_in, d_out, stream);
+}
+
 extern "C" int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
                                 int device)
 {
@@ -890,7 +898,8 @@ extern "C" int afft_plan_create(afft_plan **plan, int dtype, int rank, const int
     *plan = nullptr;
     int st = check_shape(dtype, rank, extents, 8);
     if (st) return st;
-    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL) return AFFT_ERROR_INVALID_VALUE;
+    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL && axes != AFFT_AXES_LAST2)
+        return AFFT_ERROR_INVALID_VALUE;
     Plan *p = nullptr;
     st = create_plan(p, dtype, rank, extents, axes, device);
     if (st) return st;
@@ -905,7 +914,8 @@ extern "C" int afft_plan_cached(const afft_plan **plan, int dtype, int rank, con
     *plan = nullptr;
     int st = check_shape(dtype, rank, extents, 8);
     if (st) return st;
-    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL) return AFFT_ERROR_INVALID_VALUE;
+    if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL && axes != AFFT_AXES_LAST2)
+        return AFFT_ERROR_INVALID_VALUE;
     const Plan *p = nullptr;
     st = cached_plan(p, dtype, rank, extents, axes, device);
     if (st) return st;

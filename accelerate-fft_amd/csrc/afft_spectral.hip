@@ -1,0 +1,401 @@
+// afft_spectral.hip -- fused spectral convolution over a batch of h x w images and its VJP
+// (afft_spectral_conv_fwd / afft_spectral_conv_vjp; no counterpart in the reference API).
+//
+//   forward   y_b = (1/hw) F^H (H * F x_b)       F = unnormalised 2D DFT, F^H = Reverse mode
+//   VJP       gx_b = (1/hw) F^H (conj(H) * F gy_b)
+//             gH   = (1/hw) sum_b conj(F x_b) * F gy_b      (F x_b recomputed, never stored)
+//
+// Launch structure (forward): row pass over all B*h rows (the persistent row kernel), then ONE
+// column kernel per column tile that does the column FFT, the product with H and the inverse
+// column FFT in LDS (afft_colconv), then the inverse row pass carrying the 1/hw scale.  Three
+// array passes instead of the six of the per-call composition (2D forward, product, 2D
+// inverse, scale).  The VJP column kernel (afft_colconv_vjp) transforms the columns of both
+// row-transformed arrays, accumulates conj(X) G for its group of images in registers (a
+// deterministic reduction, no atomics) and writes the back-transformed conj(H) G.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "../../include/afft.h"
+#include "afft_device.hpp"
+#include "afft_internal.hpp"
+#include "afft_plan.hpp"
+
+namespace afft {
+
+struct ColConvArgs {
+    void *data;          // B x h x w, row-transformed; column-transformed result written back
+    const void *other;   // VJP: the row-transformed x (B x h x w)
+    const void *H;       // h x w filter
+    void *partial;       // VJP: groups x h x w partial sums of conj(X) G
+    const void *tw;
+    PassTable pt;        // length h
+    int64_t B, h, w;
+    int C;               // columns per tile
+    int groups;          // VJP: images are split into `groups` contiguous groups
+};
+
+template <typename Real, int E>
+__global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_colconv(const ColConvArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    extern __shared__ __align__(16) unsigned char afft_smem[];
+    V *lds = reinterpret_cast<V *>(afft_smem);
+    const int T = (int)(a.h / E), C = a.C;
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int64_t tiles = (a.w + C - 1) / C;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t b = tile / tiles;
+    const int64_t j = (tile % tiles) * C + c;
+    const bool active = j < a.w && b < a.B;
+    V *d = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
+    const V *Hc = reinterpret_cast<const V *>(a.H) + j;
+    const V *tw = reinterpret_cast<const V *>(a.tw);
+    V reg[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) reg[i] = active ? d[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+    fft_core<E>(reg, lds + c, C, a.pt, tw, t, T);
+    // product with H, then the inverse column transform as conj(F(conj(.)))
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const V h = active ? Hc[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+        reg[i] = c_conj(c_mul(reg[i], h));
+    }
+    fft_core<E>(reg, lds + c, C, a.pt, tw, t, T);
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) d[(int64_t)(t + i * T) * a.w] = c_conj(reg[i]);
+    }
+}
+
+template <typename Real, int E>
+__global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_colconv_vjp(const ColConvArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    extern __shared__ __align__(16) unsigned char afft_smem[];
+    V *lds = reinterpret_cast<V *>(afft_smem);
+    const int T = (int)(a.h / E), C = a.C;
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int64_t tiles = (a.w + C - 1) / C;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t g = tile / tiles;
+    const int64_t j = (tile % tiles) * C + c;
+    const bool active = j < a.w;
+    const int64_t per = (a.B + a.groups - 1) / a.groups;
+    const int64_t b0 = g * per, b1 = (b0 + per < a.B) ? b0 + per : a.B;
+    const V *Hc = reinterpret_cast<const V *>(a.H) + j;
+    const V *tw = reinterpret_cast<const V *>(a.tw);
+    V acc[E];
+    V hconj[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        acc[i] = c_make<V>(0, 0);
+        hconj[i] = active ? c_conj(Hc[(int64_t)(t + i * T) * a.w]) : c_make<V>(0, 0);
+    }
+    for (int64_t b = b0; b < b1; ++b) {
+        const V *x = reinterpret_cast<const V *>(a.other) + b * a.h * a.w + j;
+        V *gy = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
+        V X[E], G[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            X[i] = active ? x[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+            G[i] = active ? gy[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+        }
+        fft_core<E>(X, lds + c, C, a.pt, tw, t, T);
+        fft_core<E>(G, lds + c, C, a.pt, tw, t, T);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            acc[i] = c_add(acc[i], c_mulc(G[i], X[i])); // conj(X) G
+            G[i] = c_conj(c_mul(G[i], hconj[i]));
+        }
+        fft_core<E>(G, lds + c, C, a.pt, tw, t, T);
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) gy[(int64_t)(t + i * T) * a.w] = c_conj(G[i]);
+        }
+    }
+    if (active) {
+        V *p = reinterpret_cast<V *>(a.partial) + g * a.h * a.w + j;
+#pragma unroll
+        for (int i = 0; i < E; ++i) p[(int64_t)(t + i * T) * a.w] = acc[i];
+    }
+}
+
+// gH[k] = scale * sum_g partial[g][k]
+template <typename Real>
+__global__ void afft_reduce_groups(typename cplx_of<Real>::type *out, const typename cplx_of<Real>::type *partial,
+                                   int groups, int64_t n, Real scale)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        V s = c_make<V>(0, 0);
+        for (int g = 0; g < groups; ++g) s = c_add(s, partial[g * n + k]);
+        out[k] = c_scale(s, scale);
+    }
+}
+
+namespace {
+
+#define AFFT_SPEC_E(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16)
+
+template <typename Real>
+hipError_t launch_colconv(bool vjp, int E, const ColConvArgs &a, int block, int64_t grid, size_t lds,
+                          hipStream_t s)
+{
+    const void *fn = nullptr;
+    switch (E) {
+#define X(EV)                                                                                    \
+    case EV:                                                                                     \
+        fn = vjp ? reinterpret_cast<const void *>(afft_colconv_vjp<Real, EV>)                   \
+                 : reinterpret_cast<const void *>(afft_colconv<Real, EV>);                      \
+        break;
+        AFFT_SPEC_E(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    void *args[] = {const_cast<ColConvArgs *>(&a)};
+    return hipLaunchKernel(fn, dim3((unsigned)grid), dim3(block), args, lds, s);
+}
+
+struct ColShape {
+    LdsLayout L;
+    int C, block;
+    size_t lds;
+};
+
+bool col_shape(int dtype, int64_t h, ColShape &cs)
+{
+    if (h > 16384 || !is_5smooth(h)) return false;
+    if (!choose_layout(h, dtype, cs.L, 16)) return false;
+    const int T = (int)(h / cs.L.E);
+    const int maxb = max_block_threads(dtype == AFFT_C128, cs.L.E);
+    const size_t esz = dtype == AFFT_C64 ? 8 : 16;
+    cs.C = 16;
+    while (cs.C > 1 && (cs.C * T > maxb || (size_t)cs.C * lds_padded_len((int)h) * esz > 80 * 1024)) cs.C /= 2;
+    if (cs.C * T > maxb) return false;
+    cs.block = cs.C * T;
+    cs.lds = cs.L.npass > 1 ? (size_t)cs.C * lds_padded_len((int)h) * esz : 0;
+    return cs.lds <= 160 * 1024;
+}
+
+// per-(device, dtype, h) column tables, kept for the life of the process
+struct ColTables {
+    Plan owner;
+    PassTable pt;
+    const void *tw = nullptr;
+};
+
+int col_tables(int dtype, int64_t h, int device, const ColShape &cs, const ColTables *&out)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int64_t>, ColTables *> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(device, dtype, h);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        out = it->second;
+        return AFFT_SUCCESS;
+    }
+    DeviceInfo dev;
+    int st = device_info(device, dev);
+    if (st) return st;
+    ColTables *ct = new ColTables();
+    ct->owner.dtype = dtype;
+    ct->owner.device = device;
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        delete ct;
+        return AFFT_ERROR_HIP;
+    }
+    st = table_pass_twiddles(ct->owner, dev, s, h, cs.L, ct->pt, ct->tw);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    if (st) {
+        free_plan(ct->owner);
+        delete ct;
+        return st;
+    }
+    cache[key] = ct;
+    out = ct;
+    return AFFT_SUCCESS;
+}
+
+// Unfused composition for shapes without a fused column layout: batched 2D plans plus pointwise
+// kernels.  Same mathematics, more array passes.
+int conv_fwd_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, const void *H, void *y,
+                     int dev, hipStream_t s)
+{
+    const int64_t ext[3] = {B, h, w};
+    const Plan *p2 = nullptr;
+    int st = cached_plan(p2, dtype, 3, ext, AFFT_AXES_LAST2, dev);
+    if (st) return st;
+    if ((st = exec_plan(*p2, -1, x, y, 1.0, s))) return st;
+    if (pointwise_mul(dtype, y, y, H, B * h * w, h * w, 0, 1.0, s) != hipSuccess) return AFFT_ERROR_HIP;
+    return exec_plan(*p2, +1, y, y, 1.0 / (double)(h * w), s);
+}
+
+int conv_vjp_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, const void *H,
+                     const void *gy, void *gx, void *gH, int dev, hipStream_t s)
+{
+    const int64_t ext[3] = {B, h, w};
+    const Plan *p2 = nullptr;
+    int st = cached_plan(p2, dtype, 3, ext, AFFT_AXES_LAST2, dev);
+    if (st) return st;
+    const size_t esz = dtype == AFFT_C64 ? 8 : 16;
+    const int64_t n = B * h * w, m = h * w;
+    void *ws = nullptr;
+    if (hipMallocAsync(&ws, 2 * (size_t)n * esz, s) != hipSuccess) return AFFT_ERROR_ALLOC;
+    void *X = ws, *G = (char *)ws + (size_t)n * esz;
+    const double scale = 1.0 / (double)m;
+    if (!(st = exec_plan(*p2, -1, x, X, 1.0, s))) st = exec_plan(*p2, -1, gy, G, 1.0, s);
+    if (!st && gH) {
+        // X <- G * conj(X), then the batch sum
+        if (pointwise_mul(dtype, X, G, X, n, n, 1, 1.0, s) != hipSuccess) st = AFFT_ERROR_HIP;
+        const int64_t g = std::min<int64_t>(4096, (m + 255) / 256);
+        if (!st) {
+            if (dtype == AFFT_C64)
+                hipLaunchKernelGGL(afft_reduce_groups<float>, dim3((unsigned)g), dim3(256), 0, s, (float2 *)gH,
+                                   (const float2 *)X, (int)B, m, (float)scale);
+            else
+                hipLaunchKernelGGL(afft_reduce_groups<double>, dim3((unsigned)g), dim3(256), 0, s, (double2 *)gH,
+                                   (const double2 *)X, (int)B, m, scale);
+            if (hipGetLastError() != hipSuccess) st = AFFT_ERROR_HIP;
+        }
+    }
+    if (!st && gx) {
+        if (pointwise_mul(dtype, G, G, H, n, m, 1, 1.0, s) != hipSuccess) st = AFFT_ERROR_HIP;
+        if (!st) st = exec_plan(*p2, +1, G, gx, scale, s);
+    }
+    (void)hipFreeAsync(ws, s);
+    return st;
+}
+
+int check_args(int dtype, int64_t B, int64_t h, int64_t w)
+{
+    if (dtype != AFFT_C64 && dtype != AFFT_C128) return AFFT_ERROR_UNSUPPORTED_DTYPE;
+    if (B < 0 || h < 0 || w < 0) return AFFT_ERROR_INVALID_VALUE;
+    return AFFT_SUCCESS;
+}
+
+} // namespace
+} // namespace afft
+
+using namespace afft;
+
+extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                      const void *d_H, void *d_y, void *stream)
+{
+    int st = check_args(dtype, batch, h, w);
+    if (st) return st;
+    if (batch * h * w == 0) return AFFT_SUCCESS;
+    if (!d_x || !d_H || !d_y) return AFFT_ERROR_INVALID_VALUE;
+    int dev;
+    if ((st = current_device(dev))) return st;
+    ColShape cs;
+    if (!col_shape(dtype, h, cs)) return conv_fwd_unfused(dtype, batch, h, w, d_x, d_H, d_y, dev, (hipStream_t)stream);
+    const ColTables *ct = nullptr;
+    if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
+    const int64_t rows[2] = {batch * h, w};
+    const Plan *rp = nullptr;
+    if ((st = cached_plan(rp, dtype, 2, rows, AFFT_AXES_INNERMOST, dev))) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // 1. forward row transforms x -> y
+    if ((st = run_plan(*rp, -1, d_x, d_y, 1.0, stream))) return st;
+    // 2. column transform, product with H, inverse column transform, in place on y
+    ColConvArgs a{};
+    a.data = d_y;
+    a.H = d_H;
+    a.tw = ct->tw;
+    a.pt = ct->pt;
+    a.B = batch;
+    a.h = h;
+    a.w = w;
+    a.C = cs.C;
+    const int64_t grid = batch * ((w + cs.C - 1) / cs.C);
+    hipError_t e = dtype == AFFT_C64 ? launch_colconv<float>(false, cs.L.E, a, cs.block, grid, cs.lds, s)
+                                     : launch_colconv<double>(false, cs.L.E, a, cs.block, grid, cs.lds, s);
+    if (e != hipSuccess) return AFFT_ERROR_HIP;
+    // 3. inverse row transforms with the 1/(h w) scale
+    return run_plan(*rp, +1, d_y, d_y, 1.0 / (double)(h * w), stream);
+}
+
+extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                      const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
+                                      void *stream)
+{
+    int st = check_args(dtype, batch, h, w);
+    if (st) return st;
+    const size_t esz = dtype == AFFT_C64 ? 8 : 16;
+    hipStream_t s = (hipStream_t)stream;
+    if (h * w == 0) return AFFT_SUCCESS;
+    if (!d_x || !d_H || !d_gy || (!d_gx && !d_gH)) return AFFT_ERROR_INVALID_VALUE;
+    if (batch == 0) {
+        if (d_gH && hipMemsetAsync(d_gH, 0, (size_t)(h * w) * esz, s) != hipSuccess) return AFFT_ERROR_HIP;
+        return AFFT_SUCCESS;
+    }
+    int dev;
+    if ((st = current_device(dev))) return st;
+    ColShape cs;
+    if (!col_shape(dtype, h, cs)) return conv_vjp_unfused(dtype, batch, h, w, d_x, d_H, d_gy, d_gx, d_gH, dev, s);
+    const ColTables *ct = nullptr;
+    if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
+    const int64_t rows[2] = {batch * h, w};
+    const Plan *rp = nullptr;
+    if ((st = cached_plan(rp, dtype, 2, rows, AFFT_AXES_INNERMOST, dev))) return st;
+    DeviceInfo di;
+    if ((st = device_info(dev, di))) return st;
+    // groups of images per column tile: enough workgroups to cover the chip twice
+    const int64_t tiles = (w + cs.C - 1) / cs.C;
+    int groups = (int)std::min<int64_t>(batch, std::max<int64_t>(1, (2 * di.cus + tiles - 1) / tiles));
+    const size_t arr = (size_t)(batch * h * w) * esz;
+    const size_t part = (size_t)groups * (size_t)(h * w) * esz;
+    void *ws = nullptr;
+    // row-transformed x; row-transformed gy lives in gx (or in the workspace when gx is not wanted)
+    const size_t ws_bytes = arr + part + (d_gx ? 0 : arr);
+    if (hipMallocAsync(&ws, ws_bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
+    char *base = (char *)ws;
+    void *xr = base;
+    void *partial = base + arr;
+    void *gr = d_gx ? d_gx : (void *)(base + arr + part);
+    if (!(st = run_plan(*rp, -1, d_x, xr, 1.0, stream))) st = run_plan(*rp, -1, d_gy, gr, 1.0, stream);
+    if (!st) {
+        ColConvArgs a{};
+        a.data = gr;
+        a.other = xr;
+        a.H = d_H;
+        a.partial = partial;
+        a.tw = ct->tw;
+        a.pt = ct->pt;
+        a.B = batch;
+        a.h = h;
+        a.w = w;
+        a.C = cs.C;
+        a.groups = groups;
+        const int64_t grid = (int64_t)groups * tiles;
+        hipError_t e = dtype == AFFT_C64 ? launch_colconv<float>(true, cs.L.E, a, cs.block, grid, cs.lds, s)
+                                         : launch_colconv<double>(true, cs.L.E, a, cs.block, grid, cs.lds, s);
+        if (e != hipSuccess) st = AFFT_ERROR_HIP;
+    }
+    const double scale = 1.0 / (double)(h * w);
+    if (!st && d_gH) {
+        const int64_t n = h * w;
+        const int64_t g = std::min<int64_t>(4096, (n + 255) / 256);
+        if (dtype == AFFT_C64)
+            hipLaunchKernelGGL(afft_reduce_groups<float>, dim3((unsigned)g), dim3(256), 0, s, (float2 *)d_gH,
+                               (const float2 *)partial, groups, n, (float)scale);
+        else
+            hipLaunchKernelGGL(afft_reduce_groups<double>, dim3((unsigned)g), dim3(256), 0, s, (double2 *)d_gH,
+                               (const double2 *)partial, groups, n, scale);
+        if (hipGetLastError() != hipSuccess) st = AFFT_ERROR_HIP;
+    }
+    if (!st && d_gx) st = run_plan(*rp, +1, d_gx, d_gx, scale, stream);
+    (void)hipFreeAsync(ws, s);
+    return st;
+}

@@ -56,6 +56,7 @@ extern "C" {
 /* which axes a plan transforms */
 #define AFFT_AXES_INNERMOST 0 /* fft: innermost axis, outer axes are a batch */
 #define AFFT_AXES_ALL 1       /* fft1D / fft2D / fft3D: every axis           */
+#define AFFT_AXES_LAST2 2     /* batched fft2D: the two innermost axes       */
 
 typedef struct afft_plan afft_plan;
 typedef struct afft_ragged_plan afft_ragged_plan;
@@ -94,10 +95,17 @@ int afft_fft2D(int mode, int dtype, int64_t h, int64_t w, const void *d_in, void
 int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, const void *d_in, void *d_out,
                void *stream);
 
+/* fft2D over each of `batch` h x w matrices stored back to back (the two innermost axes of a
+ * batch x h x w array); Inverse divides by h*w.  Extension: the reference API has no batched 2D
+ * call (fft2D takes a DIM2 only, FFT.hs:119-122), so a batch needs one fft2D per matrix. */
+int afft_fft2D_many(int mode, int dtype, int64_t batch, int64_t h, int64_t w, const void *d_in,
+                    void *d_out, void *stream);
+
 /* ---- explicit plans --------------------------------------------------------------------- */
 
 /* Build an immutable plan: kernels chosen, twiddles generated on the device in FP64 and rounded
- * to the element type.  `device` is a HIP device ordinal.  Plans carry no stream state, so one
+ * to the element type.  `axes` is AFFT_AXES_INNERMOST, AFFT_AXES_ALL or AFFT_AXES_LAST2 (rank
+ * >= 2).  `device` is a HIP device ordinal.  Plans carry no stream state, so one
  * plan may be executed concurrently on several streams.
  * Replaces the cuFFT handle creation of LLVM/PTX.hs:137-170 and PTX/Plans.hs:40-90. */
 int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,

@@ -1,0 +1,118 @@
+"""Fused spectral convolution y = Inverse(fft2D(x) * H) and its VJP, against the oracle, against
+the reference's own Reverse mode, and against central finite differences in FP64."""
+import numpy as np
+import pytest
+import torch
+
+import afft
+import oracle as O
+from afft import Mode
+from conftest import tolerance
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, dtype, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dtype).to(device)
+
+
+def host(t):
+    return t.detach().cpu().numpy().astype(np.complex128)
+
+
+def oracle_conv(x, H):
+    return np.stack([O.fft2D(O.Mode.Inverse, O.fft2D(O.Mode.Forward, xb) * H) for xb in x])
+
+
+def oracle_vjp(x, H, gy):
+    M = H.size
+    # (1/M) F^H (conj(H) F gy): the reference's Reverse mode divided by M, i.e. Inverse
+    gx = np.stack([O.fft2D(O.Mode.Reverse, np.conj(H) * O.fft2D(O.Mode.Forward, g)) / M for g in gy])
+    gH = sum(np.conj(O.fft2D(O.Mode.Forward, xb)) * O.fft2D(O.Mode.Forward, g) for xb, g in zip(x, gy)) / M
+    return gx, gH
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("shape", [(3, 64, 96), (2, 512, 512), (5, 30, 1), (1, 1, 8), (4, 96, 64)])
+def test_spectral_conv_forward(gpu, dtype, shape):
+    B, h, w = shape
+    x = O.normal(1, shape)
+    H = O.normal(2, (h, w))
+    xd, Hd = dev(x, dtype, gpu), dev(H, dtype, gpu)
+    y = afft.spectral_conv(xd, Hd)
+    want = oracle_conv(host(xd), host(Hd))
+    err = O.rel_l2(host(y), want)
+    assert err <= tolerance(dtype == torch.complex128, h * w), err
+    # the per-call composition on the GPU gives the same result
+    plain = torch.stack([afft.fft2D(Mode.Inverse, afft.fft2D(Mode.Forward, xd[b]) * Hd) for b in range(B)])
+    assert O.rel_l2(host(y), host(plain)) <= 2 * tolerance(dtype == torch.complex128, h * w)
+    batched = afft.fft2D_many(Mode.Inverse, afft.fft2D_many(Mode.Forward, xd) * Hd)
+    assert O.rel_l2(host(batched), want) <= tolerance(dtype == torch.complex128, h * w)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("shape", [(4, 64, 64), (7, 32, 48), (64, 128, 128)])
+def test_spectral_conv_vjp_matches_oracle(gpu, dtype, shape):
+    B, h, w = shape
+    x, H, gy = O.normal(3, shape), O.normal(4, (h, w)), O.normal(5, shape)
+    xd, Hd, gd = dev(x, dtype, gpu), dev(H, dtype, gpu), dev(gy, dtype, gpu)
+    gx, gH = afft.spectral_conv_vjp(xd, Hd, gd)
+    wx, wH = oracle_vjp(host(xd), host(Hd), host(gd))
+    tol = tolerance(dtype == torch.complex128, h * w)
+    assert O.rel_l2(host(gx), wx) <= tol
+    assert O.rel_l2(host(gH), wH) <= tol * 4
+    # only one of the two outputs requested
+    gx2, none = afft.spectral_conv_vjp(xd, Hd, gd, want_H=False)
+    assert none is None and O.rel_l2(host(gx2), wx) <= tol
+    none, gH2 = afft.spectral_conv_vjp(xd, Hd, gd, want_x=False)
+    assert none is None and O.rel_l2(host(gH2), wH) <= tol * 4
+
+
+def test_spectral_conv_vjp_finite_differences_fp64(gpu):
+    # loss = sum |y - t|^2 on 4 x 64 x 64, central differences in FP64 along random directions
+    B, h, w = 4, 64, 64
+    dt = torch.complex128
+    x, H, t = dev(O.normal(6, (B, h, w)), dt, gpu), dev(O.normal(7, (h, w)), dt, gpu), dev(O.normal(8, (B, h, w)), dt, gpu)
+
+    def loss(xx, HH):
+        y = afft.spectral_conv(xx, HH)
+        return float(torch.sum(torch.abs(y - t) ** 2))
+
+    y = afft.spectral_conv(x, H)
+    gy = 2 * (y - t)  # dL/d(conj y) times 2: the torch convention
+    gx, gH = afft.spectral_conv_vjp(x, H, gy)
+    eps = 1e-6
+    for seed in range(3):
+        dx = dev(O.normal(100 + seed, (B, h, w)), dt, gpu)
+        dH = dev(O.normal(200 + seed, (h, w)), dt, gpu)
+        fd_x = (loss(x + eps * dx, H) - loss(x - eps * dx, H)) / (2 * eps)
+        fd_H = (loss(x, H + eps * dH) - loss(x, H - eps * dH)) / (2 * eps)
+        an_x = float(torch.sum(torch.conj(gx) * dx).real)
+        an_H = float(torch.sum(torch.conj(gH) * dH).real)
+        assert abs(fd_x - an_x) <= 1e-6 * max(1.0, abs(an_x)), (fd_x, an_x)
+        assert abs(fd_H - an_H) <= 1e-6 * max(1.0, abs(an_H)), (fd_H, an_H)
+
+
+def test_spectral_configuration_size(gpu):
+    # 64 x 512 x 512 Complex Float: fused forward and VJP against the per-call composition
+    B, h, w = 64, 512, 512
+    g = torch.Generator(device=gpu)
+    g.manual_seed(5)
+    x = torch.complex(torch.randn(B, h, w, generator=g, device=gpu), torch.randn(B, h, w, generator=g, device=gpu))
+    H = torch.complex(torch.randn(h, w, generator=g, device=gpu), torch.randn(h, w, generator=g, device=gpu))
+    gy = torch.complex(torch.randn(B, h, w, generator=g, device=gpu), torch.randn(B, h, w, generator=g, device=gpu))
+    y = afft.spectral_conv(x, H)
+    plain = torch.stack([afft.fft2D(Mode.Inverse, afft.fft2D(Mode.Forward, x[b]) * H) for b in range(B)])
+    tol = tolerance(False, h * w)
+    assert O.rel_l2(host(y), host(plain)) <= tol
+    gx, gH = afft.spectral_conv_vjp(x, H, gy)
+    Fx = torch.stack([afft.fft2D(Mode.Forward, x[b]) for b in range(B)])
+    Fg = torch.stack([afft.fft2D(Mode.Forward, gy[b]) for b in range(B)])
+    px = torch.stack([afft.fft2D(Mode.Inverse, torch.conj(H) * Fg[b]) for b in range(B)])
+    pH = torch.sum(torch.conj(Fx) * Fg, dim=0) / (h * w)
+    assert O.rel_l2(host(gx), host(px)) <= tol
+    assert O.rel_l2(host(gH), host(pH)) <= tol
+    # two images against the FP64 oracle
+    xs = host(x[:2])
+    want = oracle_conv(xs, host(H))
+    assert O.rel_l2(host(y[:2]), want) <= tol
