@@ -17,7 +17,11 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
              not installed), over the same 16384 x 4096 array, on every host core available.
   extras     the other configurations (four-step 2^22 row, Reverse/Inverse, fft2D 4096^2,
              fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
-             time and roofline fraction.
+             time and roofline fraction; the ragged batch and the spectral convolution also
+             against the plain per-call path on the same GPU.
+
+  --profile-headline runs only the headline plan (and a plain device copy of the same array,
+             the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
 """
 import argparse
 import json
@@ -30,6 +34,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "accelerate-fft_amd"))
 
 HBM_PEAK_GBS = 8000.0
+RAGGED_SIZES = sorted({b << k for b in (1, 3, 5) for k in range(4, 14)} | {257, 1009, 4099})
+
+
+def ragged_lengths(count=10000, seed=4):
+    """Lengths drawn log-uniformly from [16, 40960] and snapped (in log distance) to the size set
+    {2^k, 3*2^k, 5*2^k : k = 4..13} + {257, 1009, 4099}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(math.log(16), math.log(40960), count)
+    logs = np.log(np.array(RAGGED_SIZES, dtype=np.float64))
+    return [RAGGED_SIZES[int(np.argmin(np.abs(logs - v)))] for v in u]
 METRIC = "FFT GFLOP/s (5N log2N) + % HBM roofline; fwd+adjoint time vs plain, same GPU"
 
 
@@ -84,7 +99,7 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(rows, n, min_seconds=8.0, max_reps=6):
+def cpu_baseline(rows, n, min_seconds=12.0, max_reps=400):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -104,6 +119,94 @@ def cpu_baseline(rows, n, min_seconds=8.0, max_reps=6):
                       f"Complex Float array, {reps} reps in {el:.1f} s on {cores} threads"}
 
 
+def ragged_extras(afft, Mode, torch, dev, args):
+    """10,000 Complex Float transforms of mixed lengths: one persistent launch (eager and as a
+    replayed hipGraph) against one plain call per transform and one batched call per length."""
+    import ctypes
+    lengths = ragged_lengths()
+    total = sum(lengths)
+    x = uniform_c((total,), torch.complex64, 4, dev)
+    out = torch.empty_like(x)
+    plan = afft.RaggedPlan(lengths, torch.complex64)
+    flops = sum(5.0 * n * math.log2(n) for n in lengths)
+    res = {"ragged_total_elements": total}
+    t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(5, args.steps // 2), 3)
+    tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(5, args.steps // 2), 3)
+    # plain: one fft1D call per transform, straight through the C ABI (no tensor allocation)
+    lib = afft._lib.lib()
+    offs = [0]
+    for n in lengths[:-1]:
+        offs.append(offs[-1] + n)
+    xp, op = x.data_ptr(), out.data_ptr()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    args_list = [(n, ctypes.c_void_p(xp + 8 * o), ctypes.c_void_p(op + 8 * o)) for n, o in zip(lengths, offs)]
+
+    def per_call():
+        for n, a, b in args_list:
+            lib.afft_fft1D(0, 0, n, a, b, s)
+
+    tp = time_calls(per_call, 3, 1)
+    # per length: the rows of each length gathered contiguously (gather not timed), one call each
+    by = {}
+    for n, o in zip(lengths, offs):
+        by.setdefault(n, []).append(o)
+    groups = [(n, torch.stack([x[o:o + n] for o in os_])) for n, os_ in by.items()]
+
+    def per_length():
+        for n, g in groups:
+            afft.fft(Mode.Forward, g)
+
+    tl = time_calls(per_length, max(3, args.steps // 5), 1)
+    res["ragged_persistent"] = {"us": round(t * 1e6, 1), "gflops": round(flops / t / 1e9, 1),
+                                "hbm_frac": round(16.0 * total / t / 1e9 / HBM_PEAK_GBS, 4)}
+    res["ragged_persistent_graph"] = {"us": round(tg * 1e6, 1), "gflops": round(flops / tg / 1e9, 1)}
+    res["ragged_plain_per_call"] = {"us": round(tp * 1e6, 1), "calls": len(lengths)}
+    res["ragged_per_length_batched"] = {"us": round(tl * 1e6, 1), "calls": len(groups)}
+    res["ragged_speedup_vs_per_call"] = round(tp / min(t, tg), 2)
+    res["ragged_speedup_vs_per_length"] = round(tl / min(t, tg), 2)
+    plan.close()
+    return res
+
+
+def spectral_extras(afft, Mode, torch, dev, args):
+    """y = Inverse(fft2D(x) * H) over 64 x 512 x 512 Complex Float, forward + VJP: the fused
+    kernels against the plain per-call composition (one fft2D per image, separate product,
+    conjugate-product, batch-sum and scale kernels)."""
+    B, h, w = 64, 512, 512
+    x = normal_c((B, h, w), torch.complex64, 5, dev)
+    H = normal_c((h, w), torch.complex64, 6, dev)
+    t = normal_c((B, h, w), torch.complex64, 7, dev)
+    M = h * w
+
+    def fused():
+        y = afft.spectral_conv(x, H)
+        gy = 2 * (y - t)
+        return afft.spectral_conv_vjp(x, H, gy)
+
+    def plain():
+        ys = []
+        for b in range(B):
+            ys.append(afft.fft2D(Mode.Reverse, afft.fft2D(Mode.Forward, x[b]) * H) * (1.0 / M))
+        y = torch.stack(ys)
+        gy = 2 * (y - t)
+        gxs, gH = [], torch.zeros_like(H)
+        for b in range(B):
+            Fx = afft.fft2D(Mode.Forward, x[b])
+            Fg = afft.fft2D(Mode.Forward, gy[b])
+            gxs.append(afft.fft2D(Mode.Reverse, torch.conj(H) * Fg) * (1.0 / M))
+            gH += torch.conj(Fx) * Fg
+        return torch.stack(gxs), gH * (1.0 / M)
+
+    tf = time_calls(fused, max(3, args.steps // 10), 2)
+    tp = time_calls(plain, 3, 1)
+    tfwd = time_calls(lambda: afft.spectral_conv(x, H), max(3, args.steps // 5), 2)
+    return {"spectral_conv_fused_fwd_vjp": {"us": round(tf * 1e6, 1)},
+            "spectral_conv_fused_fwd_only": {"us": round(tfwd * 1e6, 1),
+                                             "hbm_frac_per_pass_3": round(3 * 2 * 8.0 * B * M / tfwd / 1e9 / HBM_PEAK_GBS, 4)},
+            "spectral_conv_plain_fwd_vjp": {"us": round(tp * 1e6, 1)},
+            "spectral_conv_speedup_fwd_vjp": round(tp / tf, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +214,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--profile-headline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -130,6 +234,15 @@ def main():
 
     def step():
         plan.execute(x, -1, 1.0, out=out)
+
+    if args.profile_headline:
+        for _ in range(args.warmup + args.steps):
+            step()
+        for _ in range(args.steps):
+            out.copy_(x)  # plain copy of the same 512 MiB: counter calibration
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_headline": True, "steps": args.steps}), flush=True)
+        return
 
     # headline: K timed steps between barriers (synchronize) on both sides
     for _ in range(args.warmup):
@@ -161,7 +274,7 @@ def main():
                    "rows": rows, "n": n, "mode": "Forward", "launches_per_step": nl},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic("afft_stockham_f32_e16_fft_16384x4096_traffic.txt"),
+                     "traffic": read_traffic("afft_rows_f32_4096_fft_16384x4096.txt"),
                      "kernel_avg_us": round(kern * 1e6, 2)},
     }
     del plan
@@ -194,6 +307,8 @@ def main():
         add("fft3D_256x256x256_c128_Forward", lambda: afft.fft3D(Mode.Forward, c3), N, 32.0 * N, 5.0 * N * 24)
         del c3
         torch.cuda.empty_cache()
+        extras.update(ragged_extras(afft, Mode, torch, dev, args))
+        extras.update(spectral_extras(afft, Mode, torch, dev, args))
         line["extras"] = extras
 
     if not args.no_cpu:
