@@ -109,7 +109,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
             const int k = t + i * T;
             V v = reg[i];
             if (a.post_L) {
-                const int64_t e = (jl * (int64_t)k) % a.post_L;
+                // jl < n2 and k < n1, so jl * k < n1 * n2 = post_L: no reduction needed
+                const int64_t e = jl * (int64_t)k;
                 v = c_mul(v, power_lookup(reinterpret_cast<const V *>(a.post_hi),
                                           reinterpret_cast<const V *>(a.post_lo), e, a.post_shift));
             }

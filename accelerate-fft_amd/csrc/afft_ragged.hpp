@@ -40,21 +40,28 @@ struct RaggedClass {
 
 struct RaggedItem {
     int cls;
-    int first;  // first index into `members` (transform ids of the class, in class order)
+    int first;  // first index into `moffs` (the transforms of the class, in class order)
     int count;
+    int stage;  // long transforms: the stage being run (set by the kernel)
+    int tile;   // long transforms: the column tile being run (set by the kernel)
+    int big;    // four-step Bluestein transforms: scratch slot
 };
+
+// stages of a long transform: A = column transforms of length L1 (twiddled) into scratch,
+// B = column transforms of length L2 out of it.  Four-step Bluestein: A1 (chirp on load),
+// B1 (filter on store), A2, B2 (chirp on store).
+constexpr int kStagesTwo = 2;
+constexpr int kStagesBlue = 4;
 
 struct RaggedArgs {
     const void *in;
     void *out;
     const RaggedClass *classes;
     const RaggedItem *items;
+    const int64_t *moffs;   // element offset of every transform, in class-member order
     int nitems;
-    const int64_t *offsets; // element offset of every transform
-    const int *members;
-    unsigned *queue;        // [0] next item ticket, [1] workgroups finished
-    void *scratch;          // 2 * scratch_elems per resident workgroup
-    int64_t scratch_elems;
+    const int64_t *big_off; // scratch offset (elements) of every four-step Bluestein transform
+    void *scratch;
     int conj;               // Reverse / Inverse
     int inverse;            // divide every transform by its own length
 };

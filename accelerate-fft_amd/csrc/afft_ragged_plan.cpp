@@ -28,11 +28,10 @@ struct RaggedPlan {
     Plan tables;                 // owns every device allocation
     RaggedClass *d_classes = nullptr;
     RaggedItem *d_items = nullptr;
-    int64_t *d_offsets = nullptr;
-    int *d_members = nullptr;
-    unsigned *d_queue = nullptr;
+    int64_t *d_offsets = nullptr; // member offsets in class order
+    int64_t *d_big_off = nullptr;
     void *d_scratch = nullptr;
-    int64_t scratch_elems = 0;
+    bool has_long = false;
     int nitems = 0;
     int64_t grid = 0;
     size_t lds = 0;
@@ -72,7 +71,6 @@ struct Builder {
     int status = AFFT_SUCCESS;
     size_t budget;
     size_t lds_need = 0;
-    int64_t scratch_need = 0;
 
     bool sub(int64_t len, RaggedSub &S)
     {
@@ -102,6 +100,8 @@ struct Builder {
             if (Lx % d) continue;
             LdsLayout a{}, b{};
             if (!ragged_layout(d, rp.dtype, a) || !ragged_layout(Lx / d, rp.dtype, b)) continue;
+            // balanced factors: measured faster than fewest-pass splits, whose long strided
+            // columns coalesce badly
             const double sc = std::fabs(std::log((double)d) - std::log((double)(Lx / d)));
             if (sc < best) {
                 best = sc;
@@ -121,7 +121,6 @@ struct Builder {
             return false;
         }
         K.shift = sh;
-        scratch_need = std::max(scratch_need, Lx);
         return true;
     }
 
@@ -209,7 +208,10 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         RaggedItem it;
         double cost;
     };
+    // one item per tile of short transforms and one per long transform, longest first
     std::vector<Tmp> items;
+    std::vector<int64_t> big_off;
+    int64_t scratch = 0;
     for (auto &kv : by_len) {
         RaggedClass K{};
         if (!b.make_class(kv.first, K)) return b.status ? b.status : AFFT_ERROR_UNSUPPORTED_SIZE;
@@ -218,14 +220,26 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         const int first = (int)members.size();
         members.insert(members.end(), kv.second.begin(), kv.second.end());
         const int cnt = (int)kv.second.size();
-        const double len = (double)(K.kind == kDirect || K.kind == kTwoPhase ? K.L : K.M);
-        const double per = len * std::log2(std::max(2.0, len)) *
-                           (K.kind == kBlue || K.kind == kBlueTwo ? 2.5 : 1.0) *
-                           (K.kind >= kTwoPhase ? 1.6 : 1.0);
-        const int step = (K.kind == kDirect || K.kind == kBlue) ? K.C : 1;
-        for (int j = 0; j < cnt; j += step) {
-            const int c = std::min(step, cnt - j);
-            items.push_back({{cls, first + j, c}, per * c});
+        if (K.kind == kDirect || K.kind == kBlue) {
+            const double len = (double)(K.kind == kDirect ? K.L : K.M);
+            const double per = len * std::log2(std::max(2.0, len)) * (K.kind == kBlue ? 2.5 : 1.0);
+            for (int j = 0; j < cnt; j += K.C) {
+                const int c = std::min(K.C, cnt - j);
+                items.push_back({{cls, first + j, c, 0, 0, 0}, per * c});
+            }
+            continue;
+        }
+        rp.has_long = true;
+        const double len = (double)K.Lx;
+        const double per = len * std::log2(len) * (K.kind == kBlueTwo ? 2.5 : 1.0) * 1.6;
+        for (int j = 0; j < cnt; ++j) {
+            int big = 0;
+            if (K.kind == kBlueTwo) {
+                big = (int)big_off.size();
+                big_off.push_back(scratch);
+                scratch += 2 * K.Lx;
+            }
+            items.push_back({{cls, first + j, 1, 0, 0, big}, per});
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
@@ -236,18 +250,14 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
     int st = AFFT_SUCCESS;
     rp.d_classes = upload(rp, dev, classes, s, st);
     if (!st) rp.d_items = upload(rp, dev, its, s, st);
-    if (!st) rp.d_offsets = upload(rp, dev, offsets, s, st);
-    if (!st) rp.d_members = upload(rp, dev, members, s, st);
-    if (!st) rp.d_queue = (unsigned *)plan_alloc(rp.tables, dev, 64, st);
-    if (!st && hipMemsetAsync(rp.d_queue, 0, 64, s) != hipSuccess) st = AFFT_ERROR_HIP;
+    std::vector<int64_t> moffs(members.size());
+    for (size_t i = 0; i < members.size(); ++i) moffs[i] = offsets[members[i]];
+    if (!st) rp.d_offsets = upload(rp, dev, moffs, s, st);
+    if (!st) rp.d_big_off = upload(rp, dev, big_off, s, st);
+    if (!st && scratch > 0) rp.d_scratch = plan_alloc(rp.tables, dev, scratch * (int64_t)esz_of(rp.dtype), st);
     if (st) return st;
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
     rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
-    rp.scratch_elems = b.scratch_need;
-    if (rp.scratch_elems > 0) {
-        rp.d_scratch = plan_alloc(rp.tables, dev, rp.grid * 2 * rp.scratch_elems * (int64_t)esz_of(rp.dtype), st);
-        if (st) return st;
-    }
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
@@ -258,14 +268,12 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
     RaggedArgs a{};
     a.in = in;
     a.out = out;
+    a.nitems = rp.nitems;
+    a.big_off = rp.d_big_off;
     a.classes = rp.d_classes;
     a.items = rp.d_items;
-    a.nitems = rp.nitems;
-    a.offsets = rp.d_offsets;
-    a.members = rp.d_members;
-    a.queue = rp.d_queue;
+    a.moffs = rp.d_offsets;
     a.scratch = rp.d_scratch;
-    a.scratch_elems = rp.scratch_elems;
     a.conj = mode == AFFT_FORWARD ? 0 : 1;
     a.inverse = mode == AFFT_INVERSE ? 1 : 0;
     return launch_ragged(rp.dtype, a, rp.grid, rp.lds, s) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_HIP;
@@ -324,7 +332,22 @@ extern "C" int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const vo
     int old = -1;
     (void)hipGetDevice(&old);
     if (old != rp->device && hipSetDevice(rp->device) != hipSuccess) return AFFT_ERROR_HIP;
-    const int st = ragged_launch(*rp, mode, d_in, d_out, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    const void *src = d_in;
+    void *tmp = nullptr;
+    // stage A of a long transform writes transposed columns into the output: in place, the
+    // input is read from a private copy
+    if (d_in == d_out && rp->has_long) {
+        const size_t bytes = (size_t)rp->total * (rp->dtype == AFFT_C64 ? 8 : 16);
+        if (hipMallocAsync(&tmp, bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
+        if (hipMemcpyAsync(tmp, d_in, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            (void)hipFreeAsync(tmp, s);
+            return AFFT_ERROR_HIP;
+        }
+        src = tmp;
+    }
+    const int st = ragged_launch(*rp, mode, src, d_out, s);
+    if (tmp) (void)hipFreeAsync(tmp, s);
     if (old >= 0 && old != rp->device) (void)hipSetDevice(old);
     return st;
 }
@@ -356,6 +379,7 @@ extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const vo
             hipStream_t cap;
             if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) st = AFFT_ERROR_HIP;
             hipGraph_t g = nullptr;
+            if (!st && d_in == d_out && rp->has_long) st = AFFT_ERROR_INVALID_VALUE; // needs a copy: use afft_ragged_exec
             if (!st && hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) st = AFFT_ERROR_HIP;
             if (!st) {
                 const int lst = ragged_launch(*rp, mode, d_in, d_out, cap);
