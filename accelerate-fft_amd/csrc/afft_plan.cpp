@@ -32,6 +32,10 @@ namespace afft {
 
 static const int kRadices[] = {16, 8, 5, 4, 3, 2};
 
+static thread_local int tl_row_variant = -1;
+void force_row_variant(int v) { tl_row_variant = v; }
+int forced_row_variant() { return tl_row_variant; }
+
 static bool getenv_flag(const char *name)
 {
     const char *v = std::getenv(name);
@@ -418,8 +422,13 @@ struct Builder {
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
         if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS")) {
-            int variant = 0;
-            if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
+            // the variant being timed by the autotuner, else an explicit AFFT_ROW_VARIANT, else
+            // the autotuned choice, else the default layout
+            int variant = forced_row_variant();
+            if (variant < 0) {
+                if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
+                else variant = tuned_row_variant(plan.dtype, n, plan.device);
+            }
             if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
             add_rows(n, ntrans, src, dst, variant);
             return;

@@ -81,5 +81,10 @@ int table_power(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t L, in
 int table_bluestein(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n, int64_t M,
                     const void *&chirp, const void *&filt);
 bool is_5smooth(int64_t n);
+int tuned_row_variant(int dtype, int64_t n, int device);
+// Row-kernel variant forced for plans built by the calling thread (-1 = none): the autotuner's
+// hook for timing each candidate.
+void force_row_variant(int v);
+int forced_row_variant();
 
 } // namespace afft

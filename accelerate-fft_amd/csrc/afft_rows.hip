@@ -259,24 +259,25 @@ struct RowEntry {
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
-    AFFT_ROW(float, 0, 256, 16, true, 3, 16, 16),
-    AFFT_ROW(float, 0, 512, 16, true, 3, 16, 16, 2),
-    AFFT_ROW(float, 0, 1024, 16, true, 3, 16, 16, 4),
-    AFFT_ROW(float, 0, 2048, 16, true, 3, 16, 16, 8),
+    AFFT_ROW(float, 0, 256, 16, true, 2, 16, 16),
+    AFFT_ROW(float, 0, 512, 16, true, 2, 16, 16, 2),
+    AFFT_ROW(float, 0, 1024, 16, true, 2, 16, 16, 4),
+    AFFT_ROW(float, 0, 1024, 16, false, 3, 16, 16, 4),
+    AFFT_ROW(float, 0, 2048, 16, true, 2, 16, 16, 8),
+    AFFT_ROW(float, 0, 2048, 16, false, 3, 16, 16, 8),
     AFFT_ROW(float, 0, 4096, 16, true, 2, 16, 16, 16),
     AFFT_ROW(float, 0, 4096, 16, true, 3, 16, 16, 16),
     AFFT_ROW(float, 0, 4096, 16, false, 3, 16, 16, 16),
-    AFFT_ROW(float, 0, 4096, 16, false, 4, 16, 16, 16),
-    AFFT_ROW(float, 0, 4096, 32, true, 2, 16, 16, 16),
-    AFFT_ROW(float, 0, 4096, 32, false, 2, 16, 16, 16),
-    AFFT_ROW(float, 0, 8192, 16, true, 3, 16, 16, 16, 2),
-    AFFT_ROW(float, 0, 16384, 16, false, 4, 16, 16, 16, 4),
-    AFFT_ROW(double, 1, 256, 16, true, 2, 16, 16),
-    AFFT_ROW(double, 1, 512, 16, true, 2, 16, 16, 2),
-    AFFT_ROW(double, 1, 1024, 16, true, 2, 16, 16, 4),
-    AFFT_ROW(double, 1, 2048, 16, true, 2, 16, 16, 8),
-    AFFT_ROW(double, 1, 4096, 16, true, 2, 16, 16, 16),
-    AFFT_ROW(double, 1, 4096, 16, false, 2, 16, 16, 16),
+    AFFT_ROW(float, 0, 8192, 16, true, 2, 16, 16, 16, 2),
+    AFFT_ROW(float, 0, 8192, 16, false, 2, 16, 16, 16, 2),
+    AFFT_ROW(double, 1, 256, 8, true, 2, 8, 8, 4),
+    AFFT_ROW(double, 1, 512, 8, true, 2, 8, 8, 8),
+    AFFT_ROW(double, 1, 1024, 8, true, 2, 8, 8, 8, 2),
+    AFFT_ROW(double, 1, 1024, 8, false, 2, 8, 8, 8, 2),
+    AFFT_ROW(double, 1, 2048, 8, true, 2, 8, 8, 8, 4),
+    AFFT_ROW(double, 1, 2048, 8, false, 2, 8, 8, 8, 4),
+    AFFT_ROW(double, 1, 4096, 8, true, 2, 8, 8, 8, 8),
+    AFFT_ROW(double, 1, 4096, 8, false, 2, 8, 8, 8, 8),
 };
 
 const RowEntry *find_row(int dtype, int n, int variant)

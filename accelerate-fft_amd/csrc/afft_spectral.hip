@@ -71,8 +71,12 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     }
 }
 
+// Holds three transforms' worth of registers per thread (x, gy and the running sum): compiled
+// for workgroups of at most kVjpBlock threads so the register budget covers them.
+constexpr int kVjpBlock = 256;
+
 template <typename Real, int E>
-__global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_colconv_vjp(const ColConvArgs a)
+__global__ void __launch_bounds__(kVjpBlock) afft_colconv_vjp(const ColConvArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
@@ -89,12 +93,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     const V *Hc = reinterpret_cast<const V *>(a.H) + j;
     const V *tw = reinterpret_cast<const V *>(a.tw);
     V acc[E];
-    V hconj[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-        acc[i] = c_make<V>(0, 0);
-        hconj[i] = active ? c_conj(Hc[(int64_t)(t + i * T) * a.w]) : c_make<V>(0, 0);
-    }
+    for (int i = 0; i < E; ++i) acc[i] = c_make<V>(0, 0);
     for (int64_t b = b0; b < b1; ++b) {
         const V *x = reinterpret_cast<const V *>(a.other) + b * a.h * a.w + j;
         V *gy = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
@@ -109,7 +109,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             acc[i] = c_add(acc[i], c_mulc(G[i], X[i])); // conj(X) G
-            G[i] = c_conj(c_mul(G[i], hconj[i]));
+            const V h = active ? Hc[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+            G[i] = c_conj(c_mulc(G[i], h));             // conj(conj(H) G)
         }
         fft_core<E>(G, lds + c, C, a.pt, tw, t, T);
         if (active) {
@@ -170,12 +171,12 @@ struct ColShape {
     size_t lds;
 };
 
-bool col_shape(int dtype, int64_t h, ColShape &cs)
+bool col_shape(int dtype, int64_t h, ColShape &cs, bool vjp)
 {
     if (h > 16384 || !is_5smooth(h)) return false;
-    if (!choose_layout(h, dtype, cs.L, 16)) return false;
+    if (!choose_layout(h, dtype, cs.L, 16, vjp ? kVjpBlock : -1)) return false;
     const int T = (int)(h / cs.L.E);
-    const int maxb = max_block_threads(dtype == AFFT_C128, cs.L.E);
+    const int maxb = vjp ? kVjpBlock : max_block_threads(dtype == AFFT_C128, cs.L.E);
     const size_t esz = dtype == AFFT_C64 ? 8 : 16;
     cs.C = 16;
     while (cs.C > 1 && (cs.C * T > maxb || (size_t)cs.C * lds_padded_len((int)h) * esz > 80 * 1024)) cs.C /= 2;
@@ -195,9 +196,9 @@ struct ColTables {
 int col_tables(int dtype, int64_t h, int device, const ColShape &cs, const ColTables *&out)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int64_t>, ColTables *> cache;
+    static std::map<std::tuple<int, int, int64_t, int>, ColTables *> cache;
     std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_tuple(device, dtype, h);
+    auto key = std::make_tuple(device, dtype, h, cs.L.E);
     auto it = cache.find(key);
     if (it != cache.end()) {
         out = it->second;
@@ -299,7 +300,7 @@ extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64
     int dev;
     if ((st = current_device(dev))) return st;
     ColShape cs;
-    if (!col_shape(dtype, h, cs)) return conv_fwd_unfused(dtype, batch, h, w, d_x, d_H, d_y, dev, (hipStream_t)stream);
+    if (!col_shape(dtype, h, cs, false)) return conv_fwd_unfused(dtype, batch, h, w, d_x, d_H, d_y, dev, (hipStream_t)stream);
     const ColTables *ct = nullptr;
     if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
     const int64_t rows[2] = {batch * h, w};
@@ -343,7 +344,7 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
     int dev;
     if ((st = current_device(dev))) return st;
     ColShape cs;
-    if (!col_shape(dtype, h, cs)) return conv_vjp_unfused(dtype, batch, h, w, d_x, d_H, d_gy, d_gx, d_gH, dev, s);
+    if (!col_shape(dtype, h, cs, true)) return conv_vjp_unfused(dtype, batch, h, w, d_x, d_H, d_gy, d_gx, d_gH, dev, s);
     const ColTables *ct = nullptr;
     if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
     const int64_t rows[2] = {batch * h, w};

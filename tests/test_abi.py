@@ -106,3 +106,30 @@ def test_argument_checks_before_any_device_work():
     assert lib.afft_fft2D(0, 1, 0, 5, None, None, None) == 0
     assert lib.afft_fft_vjp(2, 0, 0, 1, ext, None, None, None) == 2   # fft2D needs rank 2
     assert lib.afft_fft_vjp(9, 0, 0, 1, ext, None, None, None) == 1
+
+
+def test_tune_table_file_handling(tmp_path):
+    lib = L.lib()
+    # save writes the header even for an empty table
+    p = tmp_path / "t.txt"
+    assert lib.afft_tune_save(str(p).encode()) == 0
+    assert p.read_text().startswith("afft-tune 1")
+    # a corrupt file is refused with an I/O status, a missing one too
+    bad = tmp_path / "bad.txt"
+    bad.write_text("not a tune file\n")
+    assert lib.afft_tune_load(str(bad).encode()) == 8
+    assert lib.afft_tune_load(str(tmp_path / "missing.txt").encode()) == 8
+    # malformed and stale lines are skipped without error
+    ok = tmp_path / "ok.txt"
+    ok.write_text("afft-tune 1\n"
+                  "gfx950 70226015 rows 0 4096 1 190.0\n"
+                  "gfx950 70226015 rows 0 4096 999 1.0\n"
+                  "garbage line\n"
+                  "gfx950 70226015 rows 7 4096 0 1.0\n")
+    assert lib.afft_tune_load(str(ok).encode()) == 0
+    out = tmp_path / "out.txt"
+    assert lib.afft_tune_save(str(out).encode()) == 0
+    lines = out.read_text().splitlines()
+    assert lines[0] == "afft-tune 1"
+    assert any(l.split()[:6] == ["gfx950", "70226015", "rows", "0", "4096", "1"] for l in lines[1:])
+    assert not any(" 999 " in l for l in lines)
