@@ -234,3 +234,14 @@ def test_large_sizes_against_numpy_pocketfft():
     assert O.rel_l2(O.fft1D(O.Mode.Forward, x), np.fft.fft(x)) < 1e-15
     y = O.normal(4, (64, 48, 40))
     assert O.rel_l2(O.fft3D(O.Mode.Inverse, y), np.fft.ifftn(y)) < 1e-15
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 4096, 4099])
+def test_float_variant_accuracy(n):
+    # the float recursion (used only to time a CPU path beside the GPU) stays within the FP32
+    # tolerance of the FP64 oracle
+    x = O.uniform(n + 5, (16, n))
+    xf = x.astype(np.complex64)
+    want = O.fft_rows(xf.astype(np.complex128), -1)
+    got = O.fft_rows_f32(xf, -1)
+    assert O.rel_l2(got, want) <= 1e-6 * max(1.0, math.log2(max(n, 2)))
