@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace afft {
 
 template <typename Real> struct cplx_of;
@@ -215,6 +217,118 @@ template <> struct dft<16> {
             for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
         }
     }
+};
+
+// ----------------------------------------------------------------------------------------------
+// Composite radices R = R1 * R2 in registers (used for 32 and 64).
+//   n = R2 n1 + n2, k = k1 + R1 k2:  X[k] = sum_n2 W_R^{n2 k1} W_R2^{n2 k2} sum_n1 x[n] W_R1^{n1 k1}
+// The twiddles W_R^m are compile-time constants: cos and sin of 2 pi m / R evaluated by a
+// constexpr Taylor series after reduction to [0, pi/4] (correct to double precision).
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+__host__ __device__ constexpr double taylor_sin(double x)
+{
+    double term = x, sum = x;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2 * k) * (2 * k + 1));
+        sum += term;
+    }
+    return sum;
+}
+__host__ __device__ constexpr double taylor_cos(double x)
+{
+    double term = 1, sum = 1;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x * x / ((2 * k - 1) * (2 * k));
+        sum += term;
+    }
+    return sum;
+}
+// quarter-period pieces: x in [0, pi/2], reduced to [0, pi/4] by cos x = sin(pi/2 - x)
+__host__ __device__ constexpr double cos_q(double x) { return x <= kPi / 4 ? taylor_cos(x) : taylor_sin(kPi / 2 - x); }
+__host__ __device__ constexpr double sin_q(double x) { return x <= kPi / 4 ? taylor_sin(x) : taylor_cos(kPi / 2 - x); }
+// cos and sin of 2 pi m / R, m reduced exactly in integers first
+__host__ __device__ constexpr double cos2pi(int m, int R)
+{
+    m %= R;
+    if (m < 0) m += R;
+    if (2 * m > R) m = R - m;                   // cos(2 pi - x) = cos x
+    const double th = 2 * kPi * m / R;          // [0, pi]
+    return th <= kPi / 2 ? cos_q(th) : -cos_q(kPi - th);
+}
+__host__ __device__ constexpr double sin2pi(int m, int R)
+{
+    m %= R;
+    if (m < 0) m += R;
+    double sg = 1;
+    if (2 * m > R) {                            // sin(2 pi - x) = -sin x
+        m = R - m;
+        sg = -1;
+    }
+    const double th = 2 * kPi * m / R;
+    return sg * (th <= kPi / 2 ? sin_q(th) : sin_q(kPi - th));
+}
+
+// v *= W_R^m (forward root exp(-2 pi i m / R)), trivial factors without a multiply
+template <int R, int M, typename V> __device__ __forceinline__ V twiddle_const(V v)
+{
+    using Real = decltype(V::x);
+    constexpr int m = ((M % R) + R) % R;
+    if constexpr (m == 0) return v;
+    else if constexpr (4 * m == R) return c_mul_negi(v);
+    else if constexpr (2 * m == R) return c_make<V>(-v.x, -v.y);
+    else if constexpr (4 * m == 3 * R) return c_make<V>(-v.y, v.x);
+    else {
+        constexpr Real c = (Real)cos2pi(m, R);
+        constexpr Real sn = (Real)-sin2pi(m, R);
+        return c_make<V>(v.x * c - v.y * sn, v.x * sn + v.y * c);
+    }
+}
+
+template <int R> struct dft;
+
+template <int R1, int R2> struct dft_split {
+    static constexpr int R = R1 * R2;
+    template <int N2, int K1, typename V> __device__ __forceinline__ static void tw(V (&a)[R2][R1])
+    {
+        a[N2][K1] = twiddle_const<R, N2 * K1>(a[N2][K1]);
+    }
+    template <typename V> __device__ __forceinline__ static void run(V *v)
+    {
+        V a[R2][R1];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            V t[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) t[n1] = v[R2 * n1 + n2];
+            dft<R1>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) a[n2][k1] = t[k1];
+        }
+        twiddle_all(a, std::make_integer_sequence<int, R1 * R2>{});
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            V t[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) t[n2] = a[n2][k1];
+            dft<R2>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
+        }
+    }
+    template <typename V, int... Is>
+    __device__ __forceinline__ static void twiddle_all(V (&a)[R2][R1], std::integer_sequence<int, Is...>)
+    {
+        (tw<Is / R1, Is % R1>(a), ...);
+    }
+};
+
+template <> struct dft<32> {
+    template <typename V> __device__ __forceinline__ static void run(V *v) { dft_split<8, 4>::run(v); }
+};
+template <> struct dft<64> {
+    template <typename V> __device__ __forceinline__ static void run(V *v) { dft_split<8, 8>::run(v); }
 };
 
 // ----------------------------------------------------------------------------------------------

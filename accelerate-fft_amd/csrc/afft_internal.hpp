@@ -18,6 +18,29 @@ struct Addr {
     int64_t es;
 };
 
+// offset of the first element of transform j (j = j0 + cnt0 * (j1 + cnt1 * j2))
+__device__ __forceinline__ int64_t addr_base(const Addr &a, int64_t j)
+{
+    const int64_t j0 = j % a.cnt0;
+    const int64_t q = j / a.cnt0;
+    const int64_t j1 = q % a.cnt1;
+    const int64_t j2 = q / a.cnt1;
+    return j0 * a.st0 + j1 * a.st1 + j2 * a.st2;
+}
+
+__device__ __forceinline__ int64_t addr_level(const Addr &a, int64_t j, int level)
+{
+    return level == 0 ? j % a.cnt0 : (j / a.cnt0) % a.cnt1;
+}
+
+template <typename V>
+__device__ __forceinline__ V power_lookup(const V *hi, const V *lo, int64_t e, int shift)
+{
+    const V a = hi[e >> shift];
+    const V b = lo[e & ((int64_t(1) << shift) - 1)];
+    return c_mul(a, b);
+}
+
 // One batched Stockham launch: ntrans transforms of length pt.n, C of them per workgroup.
 struct FftArgs {
     const void *in;
@@ -104,6 +127,11 @@ int row_occupancy(int dtype, int n, int variant);
 hipError_t launch_rows(int dtype, int n, int variant, const RowArgs &a, int64_t grid, hipStream_t s);
 
 // Launchers (afft_kernels.hip).  They return a hipError_t.
+// register-resident strided transforms (afft_cols.hip): one thread per transform of length
+// a.E, kRegColBlock threads per workgroup, no LDS
+bool regcol_supported(int dtype, int64_t n);
+int regcol_block();
+hipError_t launch_regcol(int dtype, const FftArgs &a, int64_t grid, hipStream_t s);
 hipError_t launch_stockham(int dtype, const FftArgs &a, int block, int64_t grid, size_t lds,
                            hipStream_t s);
 hipError_t launch_bluestein(int dtype, const BluesteinArgs &a, int block, int64_t grid, size_t lds,

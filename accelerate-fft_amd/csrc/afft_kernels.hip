@@ -17,20 +17,6 @@ namespace afft {
 
 template <int E> struct ElemsTag {};
 
-__device__ __forceinline__ int64_t addr_base(const Addr &a, int64_t j)
-{
-    const int64_t j0 = j % a.cnt0;
-    const int64_t q = j / a.cnt0;
-    const int64_t j1 = q % a.cnt1;
-    const int64_t j2 = q / a.cnt1;
-    return j0 * a.st0 + j1 * a.st1 + j2 * a.st2;
-}
-
-__device__ __forceinline__ int64_t addr_level(const Addr &a, int64_t j, int level)
-{
-    return level == 0 ? j % a.cnt0 : (j / a.cnt0) % a.cnt1;
-}
-
 // thread -> (transform slot c in the workgroup, thread t within the transform)
 __device__ __forceinline__ void split_tid(int tid, int C, int T, int lane_col, int &c, int &t)
 {
@@ -52,14 +38,6 @@ __device__ __forceinline__ V *lds_slot(V *lds, int c, int C, int n, int interlea
     }
     es = 1;
     return lds + (int64_t)c * lds_padded_len(n);
-}
-
-template <typename V>
-__device__ __forceinline__ V power_lookup(const V *hi, const V *lo, int64_t e, int shift)
-{
-    const V a = hi[e >> shift];
-    const V b = lo[e & ((int64_t(1) << shift) - 1)];
-    return c_mul(a, b);
 }
 
 // ----------------------------------------------------------------------------------------------

@@ -199,6 +199,30 @@ struct Builder {
         return buf;
     }
 
+    // Columns a strided workgroup aims for: neighbouring lanes take neighbouring columns, so C
+    // columns make one C * element-size byte segment per row; at least 16 columns (a 128-byte
+    // line in FP32), more while the workgroup stays at or under 256 threads.
+    static int strided_cols_target(int T)
+    {
+        int C = 16;
+        if (const char *v = std::getenv("AFFT_STRIDED_C")) return std::max(1, std::atoi(v));
+        while (C < 64 && 2 * C * T <= 256) C *= 2;
+        return C;
+    }
+
+    // A strided axis whose single-pass tile would read row segments shorter than 128 bytes is
+    // split into two passes of balanced length (four-step) whose tiles read whole lines.
+    bool split_strided(int64_t n, int64_t stride)
+    {
+        if (getenv_flag("AFFT_NO_COL_SPLIT")) return false;
+        LdsLayout L{};
+        if (!choose_layout(n, plan.dtype, L)) return false;
+        int C, block;
+        size_t lds;
+        shape(n, L.E, true, L.npass, stride, C, block, lds);
+        return (int64_t)C * (int64_t)esz() < 128 && stride * (int64_t)esz() >= 128;
+    }
+
     // workgroup shape for a transform of length n with T threads, strided or not
     void shape(int64_t n, int E, bool strided, int npass, int64_t ntrans, int &C, int &block, size_t &lds)
     {
@@ -210,7 +234,7 @@ struct Builder {
             if (npass > 1)
                 while (C > 1 && (size_t)C * per > 64 * 1024) C /= 2;
         } else {
-            C = 16;
+            C = strided_cols_target(T);
             while (C > 1 && (size_t)C * T > maxb) C /= 2;
             if (npass > 1)
                 while (C > 1 && (size_t)C * per > dev.lds_bytes) C /= 2;
@@ -226,6 +250,32 @@ struct Builder {
                       Slot src, Slot dst, int64_t post_L = 0, int post_level = 0,
                       const void *post_hi = nullptr, const void *post_lo = nullptr, int post_shift = 0)
     {
+        if (strided && regcol_supported(plan.dtype, n) && !getenv_flag("AFFT_NO_REGCOL")) {
+            // one thread per transform, whole in registers (afft_regcol)
+            Launch l{};
+            l.kind = Launch::kRegCol;
+            l.src = src;
+            l.dst = dst;
+            FftArgs &a = l.fa;
+            a.ntrans = ntrans;
+            a.ain = ain;
+            a.aout = aout;
+            a.pt.n = (int)n;
+            a.E = (int)n;
+            a.C = 1;
+            a.xcd_remap = !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
+            a.scale = 1.0;
+            a.post_L = post_L;
+            a.post_level = post_level;
+            a.post_hi = post_hi;
+            a.post_lo = post_lo;
+            a.post_shift = post_shift;
+            l.block = regcol_block();
+            l.lds = 0;
+            l.grid = (ntrans + l.block - 1) / l.block;
+            plan.launches.push_back(l);
+            return;
+        }
         LdsLayout L{};
         if (!choose_layout(n, plan.dtype, L)) {
             status = AFFT_ERROR_UNSUPPORTED_SIZE;
@@ -433,7 +483,7 @@ struct Builder {
             add_rows(n, ntrans, src, dst, variant);
             return;
         }
-        if (n <= limit && is_5smooth(n)) {
+        if (n <= limit && is_5smooth(n) && !(strided && split_strided(n, s_))) {
             add_stockham(n, ntrans, a, a, strided, src, dst);
             return;
         }
@@ -544,7 +594,8 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
     // some four-step passes transpose: in-place execution needs a private copy of the input
     for (const Launch &l : plan.launches)
-        if (l.src == Slot::kIn && l.kind == Launch::kStockham && l.fa.post_L) plan.inplace_needs_copy = true;
+        if (l.src == Slot::kIn && (l.kind == Launch::kStockham || l.kind == Launch::kRegCol) && l.fa.post_L)
+            plan.inplace_needs_copy = true;
     return b.status;
 }
 
@@ -596,7 +647,7 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
         const Launch &l = plan.launches[i];
         const bool first = i == 0, last = i + 1 == n;
         hipError_t e;
-        if (l.kind == Launch::kStockham) {
+        if (l.kind == Launch::kStockham || l.kind == Launch::kRegCol) {
             FftArgs a = l.fa;
             a.in = slot_ptr(l.src, src_in, d_out, plan);
             a.out = slot_ptr(l.dst, src_in, d_out, plan);
@@ -605,7 +656,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
-            e = launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
+            e = l.kind == Launch::kRegCol ? launch_regcol(plan.dtype, a, l.grid, s)
+                                          : launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
         } else if (l.kind == Launch::kRows) {
             RowArgs a = l.ra;
             a.in = slot_ptr(l.src, src_in, d_out, plan);

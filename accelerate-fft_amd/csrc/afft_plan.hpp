@@ -28,7 +28,7 @@ struct DeviceInfo {
 enum class Slot { kIn, kOut, kWork, kAux };
 
 struct Launch {
-    enum Kind { kStockham, kBluestein, kRows, kChirp } kind;
+    enum Kind { kStockham, kBluestein, kRows, kChirp, kRegCol } kind;
     Slot src, dst;
     FftArgs fa;
     BluesteinArgs ba;

@@ -104,6 +104,26 @@ def test_fft2D(gpu, dtype, shape):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(4096, 64), (2048, 32), (3072, 40), (4096, 16), (1024, 128),
+                                   (6000, 20), (4096, 4, 32)])
+def test_long_strided_axes(gpu, dtype, shape):
+    """Columns too long for a line-wide single-workgroup tile: split four-step into passes of
+    register-resident (16/32/64) or LDS transforms, each with whole-line row segments."""
+    x = O.normal(5 + sum(shape), shape)
+    xd = dev(x, dtype, gpu)
+    xr = host(xd)
+    n = int(np.prod(shape))
+    fn, ofn = (afft.fft2D, O.fft2D) if len(shape) == 2 else (afft.fft3D, O.fft3D)
+    for mode in Mode:
+        check(fn(mode, xd), ofn(OMODE[mode], xr), dtype, n, f"{shape} {mode}")
+    # in place on the output buffer (rows, then columns through the work buffer)
+    y = xd.clone()
+    p = afft.Plan(shape, dtype, "all")
+    p.execute(y, out=y)
+    check(y, O.fft_axes(xr, tuple(range(len(shape))), -1), dtype, n, f"{shape} in place")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("shape", [(16, 32, 64), (1, 1, 1), (3, 5, 7), (8, 1, 8), (11, 12, 10),
                                    (32, 32, 32), (64, 16, 48)])
 def test_fft3D(gpu, dtype, shape):
