@@ -36,6 +36,17 @@ static thread_local int tl_row_variant = -1;
 void force_row_variant(int v) { tl_row_variant = v; }
 int forced_row_variant() { return tl_row_variant; }
 
+// the buffer a four-step split moves its data to: output <-> work, aux <-> aux2
+static Slot other(Slot s)
+{
+    switch (s) {
+    case Slot::kOut: return Slot::kWork;
+    case Slot::kWork: return Slot::kOut;
+    case Slot::kAux: return Slot::kAux2;
+    default: return Slot::kAux;
+    }
+}
+
 static bool getenv_flag(const char *name)
 {
     const char *v = std::getenv(name);
@@ -245,12 +256,20 @@ struct Builder {
         lds = npass > 1 ? (size_t)C * per : 0;
     }
 
+    // Transforms j and j + 1 start at neighbouring elements for runs of at least 16, so a wave
+    // with one transform per lane reads whole lines.
+    static bool lanes_adjacent(const Addr &a)
+    {
+        return (a.cnt0 >= 16 && a.st0 == 1) || (a.cnt0 == 1 && a.cnt1 >= 16 && a.st1 == 1);
+    }
+
     // Plain single-workgroup transform (n 5-smooth and within the LDS limit).
     void add_stockham(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, bool strided,
                       Slot src, Slot dst, int64_t post_L = 0, int post_level = 0,
                       const void *post_hi = nullptr, const void *post_lo = nullptr, int post_shift = 0)
     {
-        if (strided && regcol_supported(plan.dtype, n) && !getenv_flag("AFFT_NO_REGCOL")) {
+        if (strided && regcol_supported(plan.dtype, n) && lanes_adjacent(ain) && lanes_adjacent(aout) &&
+            !getenv_flag("AFFT_NO_REGCOL")) {
             // one thread per transform, whole in registers (afft_regcol)
             Launch l{};
             l.kind = Launch::kRegCol;
@@ -374,11 +393,13 @@ struct Builder {
         post.ca.a = aout;
         post.ca.mode = 2;
         plan.launches.push_back(pre);
-        add_axis(M, 1, ntrans, Slot::kAux, Slot::kAux);
+        const Slot r1 = add_axis(M, 1, ntrans, Slot::kAux, Slot::kAux);
         if (status) return;
+        mid.src = mid.dst = r1;
         plan.launches.push_back(mid);
-        add_axis(M, 1, ntrans, Slot::kAux, Slot::kAux);
+        const Slot r2 = add_axis(M, 1, ntrans, r1, r1);
         if (status) return;
+        post.src = r2;
         plan.launches.push_back(post);
     }
 
@@ -458,9 +479,27 @@ struct Builder {
 
     // Transform along one axis: extent n, element stride s_ (product of the inner extents),
     // o outer blocks; reads slot src, writes slot dst.
-    void add_axis(int64_t n, int64_t s_, int64_t o, Slot src, Slot dst)
+    enum class AxisKind { kRows, kSingle, kBluestein, kSplit };
+
+    // How one axis of length n and element stride s_ is transformed; build_plan and add_axis
+    // both follow this decision.
+    AxisKind axis_kind(int64_t n, int64_t s_)
     {
-        if (status) return;
+        const bool strided = s_ > 1;
+        if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS"))
+            return AxisKind::kRows;
+        if (n <= dev.max_lds_len[plan.dtype] && is_5smooth(n) && !(strided && split_strided(n, s_)))
+            return AxisKind::kSingle;
+        if (!is_5smooth(n)) return AxisKind::kBluestein;
+        return AxisKind::kSplit;
+    }
+
+    // Transform one axis, reading slot src.  Single-pass kinds run in place or out of place and
+    // leave the result in `cur`; a four-step split leaves it in other(cur), since its second pass
+    // permutes the elements and cannot run in place.  Returns the slot holding the result.
+    Slot add_axis(int64_t n, int64_t s_, int64_t o, Slot src, Slot cur)
+    {
+        if (status) return cur;
         const bool strided = s_ > 1;
         const int64_t ntrans = o * s_;
         Addr a{};
@@ -471,7 +510,8 @@ struct Builder {
         a.st2 = n * s_;
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
-        if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS")) {
+        switch (axis_kind(n, s_)) {
+        case AxisKind::kRows: {
             // the variant being timed by the autotuner, else an explicit AFFT_ROW_VARIANT, else
             // the autotuned choice, else the default layout
             int variant = forced_row_variant();
@@ -480,18 +520,14 @@ struct Builder {
                 else variant = tuned_row_variant(plan.dtype, n, plan.device);
             }
             if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
-            add_rows(n, ntrans, src, dst, variant);
-            return;
+            add_rows(n, ntrans, src, cur, variant);
+            return cur;
         }
-        if (n <= limit && is_5smooth(n) && !(strided && split_strided(n, s_))) {
-            add_stockham(n, ntrans, a, a, strided, src, dst);
-            return;
+        case AxisKind::kSingle: add_stockham(n, ntrans, a, a, strided, src, cur); return cur;
+        case AxisKind::kBluestein: add_bluestein(n, ntrans, a, a, strided, src, cur); return cur;
+        case AxisKind::kSplit: break;
         }
-        if (!is_5smooth(n)) {
-            add_bluestein(n, ntrans, a, a, strided, src, dst);
-            return;
-        }
-        // four-step: n = n1 * n2, element i = i1*n2 + i2
+        // four-step: n = n1 * n2, element i = i1*n2 + i2, output k = k1 + n1*k2
         int64_t n1 = 0, n2 = 0;
         double best = 1e300;
         for (int64_t d = 2; d <= limit; ++d) {
@@ -507,7 +543,7 @@ struct Builder {
         }
         if (!n1) {
             status = AFFT_ERROR_UNSUPPORTED_SIZE;
-            return;
+            return cur;
         }
         // two-level table for W_n^e: lo[e & (2^sh - 1)], hi[e >> sh]
         int sh = 0;
@@ -516,33 +552,37 @@ struct Builder {
         const int64_t hi_cnt = (n + lo_cnt - 1) / lo_cnt;
         void *lo = alloc(lo_cnt * (int64_t)esz());
         void *hi = alloc(hi_cnt * (int64_t)esz());
-        if (status) return;
+        if (status) return cur;
         check(gen_power_table(plan.dtype, lo, n, 1, lo_cnt, s));
         check(gen_power_table(plan.dtype, hi, n, lo_cnt, hi_cnt, s));
-        // pass A: over i1 (stride n2*s_) for each (c, i2, o); twiddle W_n^{i2 k1}; write
-        // transposed to (i2*n1 + k1)*s_ + c
-        Addr ia{}, oa{};
+        const Slot next = other(cur);
+        int64_t &room = next == Slot::kWork ? plan.work_elems
+                        : next == Slot::kAux2 ? plan.aux2_elems : plan.aux_elems;
+        if (next != Slot::kOut) room = std::max(room, o * n * s_);
+        // pass A: over i1 (stride n2*s_) for each (c, i2, o), twiddled by W_n^{i2 k1}; k1 takes
+        // the position of i1, so the pass may run in place
+        Addr ia{};
         ia.cnt0 = s_;
         ia.cnt1 = n2;
         ia.st0 = 1;
         ia.st1 = s_;
         ia.st2 = n * s_;
         ia.es = n2 * s_;
-        oa = ia;
-        oa.st1 = n1 * s_;
-        oa.es = s_;
-        const Slot mid = (src == dst) ? Slot::kWork : dst;
-        if (mid == Slot::kWork) plan.work_elems = std::max(plan.work_elems, o * n * s_);
-        add_stockham(n1, o * s_ * n2, ia, oa, true, src, mid, n, 1, hi, lo, sh);
-        // pass B: over i2 (stride n1*s_) for each (c, k1, o), positions unchanged
-        Addr b{};
-        b.cnt0 = s_;
-        b.cnt1 = n1;
-        b.st0 = 1;
-        b.st1 = s_;
-        b.st2 = n * s_;
-        b.es = n1 * s_;
-        add_stockham(n2, o * s_ * n1, b, b, true, mid, dst);
+        add_stockham(n1, o * s_ * n2, ia, ia, true, src, cur, n, 1, hi, lo, sh);
+        // pass B: over i2 (stride s_) for each (c, k1, o), reading (k1*n2 + i2)*s_ + c and
+        // writing X[k1 + n1*k2] to (k1 + n1*k2)*s_ + c
+        Addr bi{}, bo{};
+        bi.cnt0 = s_;
+        bi.cnt1 = n1;
+        bi.st0 = 1;
+        bi.st1 = n2 * s_;
+        bi.st2 = n * s_;
+        bi.es = s_;
+        bo = bi;
+        bo.st1 = s_;
+        bo.es = n1 * s_;
+        add_stockham(n2, o * s_ * n1, bi, bo, true, cur, next);
+        return next;
     }
 };
 
@@ -560,21 +600,38 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     for (int i = 0; i < rank; ++i) plan.total *= extents[i];
     Builder b{plan, dev, s};
     if (plan.total == 0) return AFFT_SUCCESS;
-    // innermost axis first (reads the input), the others in place on the output
+    // Innermost axis first (it reads the input); every later pass works on the output or the
+    // work buffer.  A four-step split moves the data between the two, so the first pass writes
+    // to whichever of them makes the last pass end in the output: with one split the sequence is
+    // rows -> work, pass A in place on work, pass B work -> output, and only one intermediate
+    // array is live (a 128 MiB one stays within the 256 MB Infinity Cache).
+    auto take = [&](int ax) {
+        return (axes == AFFT_AXES_ALL) || ax == rank - 1 || (axes == AFFT_AXES_LAST2 && ax >= rank - 2);
+    };
+    int splits = 0;
+    {
+        int64_t inner = 1;
+        for (int ax = rank - 1; ax >= 0; --ax) {
+            if (take(ax) && extents[ax] > 1 && b.axis_kind(extents[ax], inner) == Builder::AxisKind::kSplit)
+                ++splits;
+            inner *= extents[ax];
+        }
+    }
     Slot src = Slot::kIn;
+    Slot cur = splits % 2 ? Slot::kWork : Slot::kOut;
+    if (splits) plan.work_elems = plan.total;
     int64_t inner = 1;
     for (int ax = rank - 1; ax >= 0; --ax) {
-        const bool take = (axes == AFFT_AXES_ALL) || ax == rank - 1 ||
-                          (axes == AFFT_AXES_LAST2 && ax >= rank - 2);
         const int64_t n = extents[ax];
-        if (take && n > 1) {
+        if (take(ax) && n > 1) {
             const int64_t o = plan.total / (n * inner);
-            b.add_axis(n, inner, o, src, Slot::kOut);
-            src = Slot::kOut;
+            cur = b.add_axis(n, inner, o, src, cur);
+            src = cur;
             if (b.status) break;
         }
         inner *= n;
     }
+    if (b.status == AFFT_SUCCESS && !plan.launches.empty() && cur != Slot::kOut) b.status = AFFT_ERROR_INTERNAL;
     if (b.status == AFFT_SUCCESS && plan.launches.empty()) {
         // every transformed extent is 1: the transform is the identity (Adhoc.hs:45); a copy
         // kernel still applies the scale
@@ -588,14 +645,16 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     if (b.status == AFFT_SUCCESS && plan.work_elems > 0) {
         plan.work = b.alloc(plan.work_elems * (int64_t)b.esz());
     }
+    if (b.status == AFFT_SUCCESS && plan.aux2_elems > 0) {
+        plan.aux2 = b.alloc(plan.aux2_elems * (int64_t)b.esz());
+    }
     if (b.status == AFFT_SUCCESS && plan.aux_elems > 0) {
         plan.aux = b.alloc(plan.aux_elems * (int64_t)b.esz());
     }
     if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
-    // some four-step passes transpose: in-place execution needs a private copy of the input
-    for (const Launch &l : plan.launches)
-        if (l.src == Slot::kIn && (l.kind == Launch::kStockham || l.kind == Launch::kRegCol) && l.fa.post_L)
-            plan.inplace_needs_copy = true;
+    // every pass that reads the input either runs in place or writes elsewhere before any
+    // pass writes the output, so in-place execution (input == output) needs no private copy
+    plan.inplace_needs_copy = false;
     return b.status;
 }
 
@@ -615,6 +674,7 @@ static void *slot_ptr(Slot s, const void *in, void *out, const Plan &plan)
     case Slot::kOut: return out;
     case Slot::kWork: return plan.work;
     case Slot::kAux: return plan.aux;
+    case Slot::kAux2: return plan.aux2;
     }
     return nullptr;
 }

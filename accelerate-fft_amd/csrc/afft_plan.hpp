@@ -25,7 +25,7 @@ struct DeviceInfo {
     int64_t max_lds_len[2];  // longest single-workgroup transform per dtype
 };
 
-enum class Slot { kIn, kOut, kWork, kAux };
+enum class Slot { kIn, kOut, kWork, kAux, kAux2 };
 
 struct Launch {
     enum Kind { kStockham, kBluestein, kRows, kChirp, kRegCol } kind;
@@ -54,6 +54,8 @@ struct Plan {
     void *work = nullptr;
     int64_t aux_elems = 0;
     void *aux = nullptr;
+    int64_t aux2_elems = 0;  // partner of aux for four-step splits inside a Bluestein transform
+    void *aux2 = nullptr;
     bool inplace_needs_copy = false;
 };
 

@@ -198,8 +198,9 @@ def test_explicit_plan_api(gpu):
     z = p.execute(y, +1, 1.0 / 4096)
     check(z, host(x), torch.complex64, 4096)
     p.close()
+    # rows, then the 4096-long columns as two register-resident passes of 64 (four-step)
     q = afft.Plan((4096, 4096), torch.complex64, "all")
-    assert q.info()["launches"] == 2
+    assert q.info()["launches"] == 3
     q.close()
 
 
