@@ -1,5 +1,5 @@
-// afft_internal.hpp -- launch descriptors shared by the planner (afft_plan.cpp) and the kernels
-// (afft_kernels.hip).  Host-visible, no device code.
+// afft_internal.hpp -- launch descriptors shared by the planner (afft_plan.cpp) and the kernels,
+// the launchers, and the device-side address helpers every batched kernel uses.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -115,6 +115,7 @@ struct RowArgs {
     int conj_out;
     double scale;
     int nt_in, nt_out;
+    const int64_t *roffs; // element offset of every row (ragged batches), or null: row * N
 };
 
 // afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,

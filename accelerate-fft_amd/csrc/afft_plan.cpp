@@ -247,8 +247,11 @@ struct Builder {
         } else {
             C = strided_cols_target(T);
             while (C > 1 && (size_t)C * T > maxb) C /= 2;
-            if (npass > 1)
+            if (npass > 1) {
                 while (C > 1 && (size_t)C * per > dev.lds_bytes) C /= 2;
+                // beyond whole lines, keep room for three workgroups per CU
+                while (C > 16 && (size_t)C * per > dev.lds_bytes / 3) C /= 2;
+            }
             // prefer at least 64 threads per workgroup when the LDS allows it
         }
         if ((int64_t)C > ntrans && ntrans > 0) C = (int)std::max<int64_t>(1, ntrans);

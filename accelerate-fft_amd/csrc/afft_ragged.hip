@@ -1,24 +1,23 @@
 // afft_ragged.hip -- ragged batches: many 1D transforms of mixed lengths, stored back to back,
-// in one persistent launch (kernel afft_ragged<Real, EMAX>).
+// in one persistent launch per phase (kernel afft_ragged<Real, EMAX>).
 //
-// The host sorts the transforms into length classes and cuts each class into work items (a
-// tile of same-length transforms, or one long transform), ordered by decreasing cost.  The
-// resident workgroups walk the device-side item list round-robin; no workgroup ever waits for
-// another, so the launch always drains, and there is no device-side state to reset between
-// launches: the same launch can be replayed as a captured hipGraph.
+// The host sorts the transforms into length classes and cuts them into work items, ordered by
+// decreasing cost within each phase.  The resident workgroups walk the phase's device-side item
+// list round-robin; no workgroup ever waits for another, so every launch drains, and there is no
+// device-side state to reset between launches: the launches replay as one captured hipGraph.
 //
 // Item kinds:
 //   kDirect     C transforms of length L <= one workgroup's layout, Stockham passes in LDS
 //   kBlue       C transforms of a non-5-smooth length with padded length M <= one workgroup:
 //               the fused chirp-z transform (two forward passes sharing one LDS slot)
-//   kTwoPhase   one long 5-smooth transform by the four-step scheme (L = L1 * L2): stage A
-//               column tiles write the twiddled transposed columns into the output, stage B
+//   kTwoPhase   one column tile of one stage of a long 5-smooth transform (four-step, L = L1 * L2):
+//               stage A tiles write the twiddled transposed columns into the output, stage B
 //               tiles transform them in place
-//   kBlueTwo    one long chirp-z transform: four such stages through two scratch buffers
-// A long transform stays on one workgroup, which runs its stages back to back: handing stages
-// to workgroups on other XCDs needs an L2 write-back per hand-off plus device-scope counters,
-// and both cost more than the parallelism they buy (measured with per-tile queue items and with
-// grid barriers between stages).
+//   kBlueTwo    one column tile of one of the four stages of a long chirp-z transform, through
+//               two scratch buffers
+// Stage s of every long transform runs in phase s, a launch of its own: the launch boundary
+// makes a stage's stores visible to every XCD, which costs far less than device-scope counters
+// or grid barriers inside one launch, and the tiles of a stage spread over the whole grid.
 #include <hip/hip_runtime.h>
 
 #include "afft_device.hpp"
@@ -212,23 +211,27 @@ __device__ __forceinline__ void tile_dispatch(const RaggedArgs &a, const RaggedC
     }
 }
 
-__device__ __forceinline__ unsigned stage_tiles(const RaggedClass &K, int stage)
-{
-    const int64_t L1 = K.s1.pt.n, L2 = K.Lx / L1;
-    return (unsigned)((stage % 2 == 0) ? (L2 + K.s1.C - 1) / K.s1.C : (L1 + K.s2.C - 1) / K.s2.C);
-}
-
 template <typename Real, int EMAX>
 __global__ void __launch_bounds__(kRaggedBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) afft_ragged(const RaggedArgs a)
 {
     extern __shared__ __align__(16) unsigned char afft_smem[];
+    // the current item's class descriptor: its pass tables are read on every pass, so they are
+    // staged in LDS once per item instead of costing a global-memory round trip per pass
+    __shared__ __align__(16) RaggedClass sK;
+    static_assert(sizeof(RaggedClass) % 4 == 0, "descriptor copied as dwords");
     {
         // items are sorted by decreasing cost; workgroup w takes w, w + G, w + 2G ... (a shared
         // ticket counter serialises on one address across the XCDs and costs more than the
         // balance it buys)
         for (int idx = blockIdx.x; idx < a.nitems; idx += gridDim.x) {
             const RaggedItem it = a.items[idx];
-            const RaggedClass &K = a.classes[it.cls];
+            {
+                const uint32_t *src = reinterpret_cast<const uint32_t *>(a.classes + it.cls);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(&sK);
+                for (int w = threadIdx.x; w < (int)(sizeof(RaggedClass) / 4); w += blockDim.x) dst[w] = src[w];
+            }
+            __syncthreads();
+            const RaggedClass &K = sK;
             if (K.kind == kDirect) {
                 switch (K.E) {
 #define AFFT_RD(EV)                                                                              \
@@ -252,22 +255,15 @@ __global__ void __launch_bounds__(kRaggedBlock) __attribute__((amdgpu_waves_per_
                 default: break;
                 }
             } else {
-                // a long transform: every tile of every stage by this workgroup, in order.  The
-                // stages hand data over inside one CU, so a workgroup barrier (stores complete
-                // to L2, L1 shared by the workgroup) is all the synchronisation they need.
-                const int nst = K.kind == kBlueTwo ? 4 : 2;
+                // count consecutive column tiles of one stage of a long transform
                 RaggedItem t = it;
-                for (int st = 0; st < nst; ++st) {
-                    t.stage = st;
-                    const unsigned tiles = stage_tiles(K, st);
-                    for (unsigned tl = 0; tl < tiles; ++tl) {
-                        t.tile = (int)tl;
-                        tile_dispatch<Real, EMAX>(a, K, t, afft_smem);
-                        __syncthreads();
-                    }
+                for (int k = 0; k < it.count; ++k) {
+                    t.tile = it.tile + k;
+                    tile_dispatch<Real, EMAX>(a, K, t, afft_smem);
+                    __syncthreads();
                 }
             }
-            __syncthreads(); // LDS reuse by the next item
+            __syncthreads(); // LDS (tiles and descriptor) reuse by the next item
         }
     }
 }

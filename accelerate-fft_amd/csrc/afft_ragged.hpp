@@ -41,9 +41,9 @@ struct RaggedClass {
 struct RaggedItem {
     int cls;
     int first;  // first index into `moffs` (the transforms of the class, in class order)
-    int count;
-    int stage;  // long transforms: the stage being run (set by the kernel)
-    int tile;   // long transforms: the column tile being run (set by the kernel)
+    int count;  // short transforms: transforms in the item; long ones: column tiles in the item
+    int stage;  // long transforms: the stage (= phase) of the item
+    int tile;   // long transforms: the first column tile of the item
     int big;    // four-step Bluestein transforms: scratch slot
 };
 

@@ -1,10 +1,11 @@
 // afft_ragged_plan.cpp -- plans for ragged batches (afft_ragged_* in include/afft.h).
 //
 // Planning: transforms are grouped by length into classes; each class gets a kernel path
-// (direct LDS transform, fused Bluestein, in-workgroup four-step, or four-step Bluestein), its
-// FP64-generated tables, and is cut into work items.  Items are ordered by decreasing estimated
-// cost (longest-processing-time first), so the long transforms start while the short ones
-// fill the tail.
+// (direct LDS transform, fused Bluestein, four-step, or four-step Bluestein), its FP64-generated
+// tables, and is cut into work items.  A long (four-step) transform becomes one item per column
+// tile of each stage, and stage p runs in phase p; the short items fill the phases to balance
+// their estimated cost.  Within a phase, items are ordered by decreasing cost
+// (longest-processing-time first).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,7 +34,11 @@ struct RaggedPlan {
     void *d_scratch = nullptr;
     bool has_long = false;
     int nitems = 0;
-    int64_t grid = 0;
+    // phase p runs stage p of every long transform; phases are separate launches, so a stage
+    // sees every store of the one before it
+    int nphase = 0;
+    int phase_off[kStagesBlue] = {}, phase_n[kStagesBlue] = {};
+    int64_t phase_grid[kStagesBlue] = {};
     size_t lds = 0;
     std::mutex graph_mu;
     std::map<std::tuple<int, const void *, void *>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
@@ -208,8 +213,9 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         RaggedItem it;
         double cost;
     };
-    // one item per tile of short transforms and one per long transform, longest first
-    std::vector<Tmp> items;
+    // short transforms: one item per tile of C transforms; long transforms: one item per
+    // column tile of every stage, in the phase of that stage
+    std::vector<Tmp> shorts, phase[kStagesBlue];
     std::vector<int64_t> big_off;
     int64_t scratch = 0;
     for (auto &kv : by_len) {
@@ -225,13 +231,14 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             const double per = len * std::log2(std::max(2.0, len)) * (K.kind == kBlue ? 2.5 : 1.0);
             for (int j = 0; j < cnt; j += K.C) {
                 const int c = std::min(K.C, cnt - j);
-                items.push_back({{cls, first + j, c, 0, 0, 0}, per * c});
+                shorts.push_back({{cls, first + j, c, 0, 0, 0}, per * c});
             }
             continue;
         }
         rp.has_long = true;
-        const double len = (double)K.Lx;
-        const double per = len * std::log2(len) * (K.kind == kBlueTwo ? 2.5 : 1.0) * 1.6;
+        const int nst = K.kind == kBlueTwo ? kStagesBlue : kStagesTwo;
+        rp.nphase = std::max(rp.nphase, nst);
+        const int64_t L1 = K.s1.pt.n, L2 = K.Lx / L1;
         for (int j = 0; j < cnt; ++j) {
             int big = 0;
             if (K.kind == kBlueTwo) {
@@ -239,12 +246,36 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
                 big_off.push_back(scratch);
                 scratch += 2 * K.Lx;
             }
-            items.push_back({{cls, first + j, 1, 0, 0, big}, per});
+            for (int st = 0; st < nst; ++st) {
+                const RaggedSub &S = st % 2 == 0 ? K.s1 : K.s2;
+                const int64_t cols = st % 2 == 0 ? L2 : L1;
+                const int tiles = (int)((cols + S.C - 1) / S.C);
+                const double len = (double)S.pt.n;
+                const double per = len * std::log2(std::max(2.0, len)) * S.C * 1.3;
+                for (int tl = 0; tl < tiles; ++tl) phase[st].push_back({{cls, first + j, 1, st, tl, big}, per});
+            }
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
+    rp.nphase = std::max(rp.nphase, 1);
+    // the short items go where the phases are lightest, largest first
+    std::stable_sort(shorts.begin(), shorts.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
+    double load[kStagesBlue] = {};
+    for (int p = 0; p < rp.nphase; ++p)
+        for (const Tmp &t : phase[p]) load[p] += t.cost;
+    for (const Tmp &t : shorts) {
+        int p = 0;
+        for (int q = 1; q < rp.nphase; ++q)
+            if (load[q] < load[p]) p = q;
+        phase[p].push_back(t);
+        load[p] += t.cost;
+    }
     std::vector<RaggedItem> its;
-    for (auto &t : items) its.push_back(t.it);
+    for (int p = 0; p < rp.nphase; ++p) {
+        std::stable_sort(phase[p].begin(), phase[p].end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
+        rp.phase_off[p] = (int)its.size();
+        rp.phase_n[p] = (int)phase[p].size();
+        for (auto &t : phase[p]) its.push_back(t.it);
+    }
     rp.nitems = (int)its.size();
     rp.lds = b.lds_need;
     int st = AFFT_SUCCESS;
@@ -257,7 +288,7 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
     if (!st && scratch > 0) rp.d_scratch = plan_alloc(rp.tables, dev, scratch * (int64_t)esz_of(rp.dtype), st);
     if (st) return st;
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
-    rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
+    for (int p = 0; p < rp.nphase; ++p) rp.phase_grid[p] = std::min<int64_t>(resident, rp.phase_n[p]);
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
@@ -268,15 +299,19 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
     RaggedArgs a{};
     a.in = in;
     a.out = out;
-    a.nitems = rp.nitems;
     a.big_off = rp.d_big_off;
     a.classes = rp.d_classes;
-    a.items = rp.d_items;
     a.moffs = rp.d_offsets;
     a.scratch = rp.d_scratch;
     a.conj = mode == AFFT_FORWARD ? 0 : 1;
     a.inverse = mode == AFFT_INVERSE ? 1 : 0;
-    return launch_ragged(rp.dtype, a, rp.grid, rp.lds, s) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_HIP;
+    for (int p = 0; p < rp.nphase; ++p) {
+        if (rp.phase_n[p] == 0) continue;
+        a.items = rp.d_items + rp.phase_off[p];
+        a.nitems = rp.phase_n[p];
+        if (launch_ragged(rp.dtype, a, rp.phase_grid[p], rp.lds, s) != hipSuccess) return AFFT_ERROR_HIP;
+    }
+    return AFFT_SUCCESS;
 }
 
 } // namespace afft
@@ -352,10 +387,10 @@ extern "C" int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const vo
     return st;
 }
 
-// Capture the launch once per (mode, in, out) on a private stream and replay the instantiated
-// graph on the caller's stream (capture is not allowed on the legacy default stream).  The graph
-// holds a single kernel node: the ticket counters are reset by the kernel itself, so there is no
-// memset node to replay.
+// Capture the phase launches once per (mode, in, out) on a private stream and replay the
+// instantiated graph on the caller's stream (capture is not allowed on the legacy default
+// stream).  The graph holds only kernel nodes: the kernel keeps no device-side state, so there
+// is no memset node to replay.
 extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
                                       void *stream)
 {

@@ -115,7 +115,7 @@ template <int N, int E> struct RowShape {
     static constexpr int BLOCK = C * T;
 };
 
-template <typename Real, int N, int E, bool PF, int W, int... Rs>
+template <typename Real, int N, int E, bool PF, int W, bool LD, int... Rs>
 __global__ void __launch_bounds__((RowShape<N, E>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W, 8)))
 afft_rows(const RowArgs a)
 {
@@ -141,11 +141,12 @@ afft_rows(const RowArgs a)
     const bool cin = a.conj_in, cout = a.conj_out;
 
     const bool nt_in = a.nt_in, nt_out = a.nt_out;
+    const int64_t *__restrict__ roffs = a.roffs;
     V nxt[E];
     auto load_tile = [&](int64_t tile) {
         const int64_t row = tile * C + c;
         if (tile < ntiles && row < nrows) {
-            const V *p = in + row * N + t;
+            const V *p = in + (roffs ? roffs[row] : row * N) + t;
             if (nt_in) {
 #pragma unroll
                 for (int i = 0; i < E; ++i) nxt[i] = c_load_nt(p + i * T);
@@ -165,7 +166,9 @@ afft_rows(const RowArgs a)
         constexpr int Ns = RL::ns(p);
         if constexpr (Ns > 1) {
             constexpr int B = E / R;
-            constexpr int NB = (T % Ns == 0) ? 1 : B;
+            // the last pass (Ns * R == N) has k = t + j*T exactly: one base W^t per thread, the
+            // factor W^{jT} of butterfly j is a compile-time constant
+            constexpr int NB = (T % Ns == 0 || Ns * R == N) ? 1 : B;
             constexpr int STEP = N / (Ns * R);
 #pragma unroll
             for (int jb = 0; jb < NB; ++jb) {
@@ -180,6 +183,11 @@ afft_rows(const RowArgs a)
     if constexpr (PF) load_tile(g);
     for (; g < ntiles; g += gridDim.x) {
         if constexpr (!PF) load_tile(g);
+        // LD: the LDS addresses depend only on the thread; passing its index through an empty
+        // asm once per row makes them recomputed here instead of hoisted out of the loop, where
+        // with many passes (or a tight register budget) they would keep many registers live
+        int tl = t;
+        if constexpr (LD) __asm__ volatile("" : "+v"(tl));
         V reg[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
@@ -194,25 +202,31 @@ afft_rows(const RowArgs a)
             constexpr int R = RL::R[p];
             constexpr int Ns = RL::ns(p);
             constexpr int B = E / R;
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
+            static_for<B>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
                 V v[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[r] = reg[j + r * B];
                 if constexpr (Ns > 1) {
-                    constexpr int NB = (T % Ns == 0) ? 1 : B;
-                    const int jb = NB == 1 ? 0 : j;
-                    apply_twiddles<R>(v, tb[p][2 * jb], tb[p][2 * jb + 1]);
+                    if constexpr (T % Ns == 0) {
+                        apply_twiddles<R>(v, tb[p][0], tb[p][1]);
+                    } else if constexpr (Ns * R == N) {
+                        const V b1 = twiddle_const<N, j * T>(tb[p][0]);
+                        const V b4 = R > 5 ? twiddle_const<N, 4 * j * T>(tb[p][1]) : b1;
+                        apply_twiddles<R>(v, b1, b4);
+                    } else {
+                        apply_twiddles<R>(v, tb[p][2 * j], tb[p][2 * j + 1]);
+                    }
                 }
                 dft<R>::run(v);
 #pragma unroll
                 for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
-            }
+            });
             if constexpr (p + 1 < NP) {
                 lds_sync(); // the previous exchange has been read by everyone
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
-                    const int b = t + j * T;
+                    const int b = tl + j * T;
                     const int k = b % Ns;
                     const int dst = (b - k) * R + k;
 #pragma unroll
@@ -220,13 +234,13 @@ afft_rows(const RowArgs a)
                 }
                 lds_sync();
 #pragma unroll
-                for (int i = 0; i < E; ++i) reg[i] = slot[lds_pad(t + i * T)];
+                for (int i = 0; i < E; ++i) reg[i] = slot[lds_pad(tl + i * T)];
             }
         });
 
         const int64_t row = g * C + c;
         if (row < nrows) {
-            V *q = out + row * N + t;
+            V *q = out + (roffs ? roffs[row] : row * N) + t;
             if (nt_out) {
 #pragma unroll
                 for (int i = 0; i < E; ++i) c_store_nt(q + i * T, c_scale(cout ? c_conj(reg[i]) : reg[i], sc));
@@ -254,30 +268,36 @@ struct RowEntry {
     int block;
 };
 
-#define AFFT_ROW(REAL, DT, N, E, PF, W, ...)                                                     \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, __VA_ARGS__>, RowShape<N, E>::BLOCK}
+#define AFFT_ROW(REAL, DT, N, E, PF, W, LD, ...)                                                 \
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, __VA_ARGS__>, RowShape<N, E>::BLOCK}
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
-    AFFT_ROW(float, 0, 256, 16, true, 2, 16, 16),
-    AFFT_ROW(float, 0, 512, 16, true, 2, 16, 16, 2),
-    AFFT_ROW(float, 0, 1024, 16, true, 2, 16, 16, 4),
-    AFFT_ROW(float, 0, 1024, 16, false, 3, 16, 16, 4),
-    AFFT_ROW(float, 0, 2048, 16, true, 2, 16, 16, 8),
-    AFFT_ROW(float, 0, 2048, 16, false, 3, 16, 16, 8),
-    AFFT_ROW(float, 0, 4096, 16, true, 2, 16, 16, 16),
-    AFFT_ROW(float, 0, 4096, 16, true, 3, 16, 16, 16),
-    AFFT_ROW(float, 0, 4096, 16, false, 3, 16, 16, 16),
-    AFFT_ROW(float, 0, 8192, 16, true, 2, 16, 16, 16, 2),
-    AFFT_ROW(float, 0, 8192, 16, false, 2, 16, 16, 16, 2),
-    AFFT_ROW(double, 1, 256, 8, true, 2, 8, 8, 4),
-    AFFT_ROW(double, 1, 512, 8, true, 2, 8, 8, 8),
-    AFFT_ROW(double, 1, 1024, 8, true, 2, 8, 8, 8, 2),
-    AFFT_ROW(double, 1, 1024, 8, false, 2, 8, 8, 8, 2),
-    AFFT_ROW(double, 1, 2048, 8, true, 2, 8, 8, 8, 4),
-    AFFT_ROW(double, 1, 2048, 8, false, 2, 8, 8, 8, 4),
-    AFFT_ROW(double, 1, 4096, 8, true, 2, 8, 8, 8, 8),
-    AFFT_ROW(double, 1, 4096, 8, false, 2, 8, 8, 8, 8),
+    AFFT_ROW(float, 0, 256, 16, true, 2, false, 16, 16),
+    AFFT_ROW(float, 0, 512, 16, true, 2, false, 16, 16, 2),
+    AFFT_ROW(float, 0, 1024, 16, true, 2, false, 16, 16, 4),
+    AFFT_ROW(float, 0, 1024, 16, false, 3, true, 16, 16, 4),
+    AFFT_ROW(float, 0, 2048, 16, true, 2, false, 16, 16, 8),
+    AFFT_ROW(float, 0, 2048, 16, false, 3, true, 16, 16, 8),
+    AFFT_ROW(float, 0, 4096, 16, true, 2, false, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 16, true, 3, true, 16, 16, 16),
+    AFFT_ROW(float, 0, 4096, 16, false, 3, true, 16, 16, 16),
+    AFFT_ROW(float, 0, 8192, 16, true, 2, false, 16, 16, 16, 2),
+    AFFT_ROW(float, 0, 8192, 16, false, 2, false, 16, 16, 16, 2),
+    // mixed radices: the odd radix last, so that every earlier pass keeps one twiddle base
+    AFFT_ROW(float, 0, 5120, 20, true, 2, true, 4, 4, 4, 4, 4, 5),
+    AFFT_ROW(float, 0, 6144, 24, true, 2, true, 4, 8, 8, 8, 3),
+    AFFT_ROW(float, 0, 10240, 20, true, 2, true, 2, 4, 4, 4, 4, 4, 5),
+    AFFT_ROW(float, 0, 12288, 24, true, 2, true, 8, 8, 8, 8, 3),
+    AFFT_ROW(float, 0, 12288, 24, false, 2, true, 8, 8, 8, 8, 3),
+    AFFT_ROW(double, 1, 256, 8, true, 2, false, 8, 8, 4),
+    AFFT_ROW(double, 1, 512, 8, true, 2, false, 8, 8, 8),
+    AFFT_ROW(double, 1, 1024, 8, true, 2, false, 8, 8, 8, 2),
+    AFFT_ROW(double, 1, 1024, 8, false, 2, false, 8, 8, 8, 2),
+    AFFT_ROW(double, 1, 2048, 8, true, 2, false, 8, 8, 8, 4),
+    AFFT_ROW(double, 1, 2048, 8, false, 2, false, 8, 8, 8, 4),
+    AFFT_ROW(double, 1, 4096, 8, true, 2, false, 8, 8, 8, 8),
+    AFFT_ROW(double, 1, 4096, 8, false, 2, false, 8, 8, 8, 8),
 };
 
 const RowEntry *find_row(int dtype, int n, int variant)

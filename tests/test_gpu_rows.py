@@ -13,7 +13,7 @@ from conftest import tolerance
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [(torch.complex64, n) for n in (256, 512, 1024, 2048, 4096, 8192)] + \
+LENGTHS = [(torch.complex64, n) for n in (256, 512, 1024, 2048, 4096, 8192, 5120, 6144, 10240, 12288)] + \
           [(torch.complex128, n) for n in (256, 512, 1024, 2048, 4096)]
 
 
