@@ -11,11 +11,14 @@ namespace afft {
 
 // Addressing of a batch of 1D transforms.  Transform j decomposes as
 //   j0 = j % cnt0,  j1 = (j / cnt0) % cnt1,  j2 = j / (cnt0 * cnt1)
-// and its element i lives at  j0*st0 + j1*st1 + j2*st2 + i*es  (in elements).
+// and its element i lives at  j0*st0 + j1*st1 + j2*st2 + i*es  (in elements), or, with a
+// gather table (ragged batches: rows of one length at arbitrary offsets), at
+// j0*st0 + j1*st1 + toff[j2] + i*es.
 struct Addr {
     int64_t cnt0, cnt1;
     int64_t st0, st1, st2;
     int64_t es;
+    const int64_t *toff;
 };
 
 // offset of the first element of transform j (j = j0 + cnt0 * (j1 + cnt1 * j2))
@@ -25,7 +28,7 @@ __device__ __forceinline__ int64_t addr_base(const Addr &a, int64_t j)
     const int64_t q = j / a.cnt0;
     const int64_t j1 = q % a.cnt1;
     const int64_t j2 = q / a.cnt1;
-    return j0 * a.st0 + j1 * a.st1 + j2 * a.st2;
+    return j0 * a.st0 + j1 * a.st1 + (a.toff ? a.toff[j2] : j2 * a.st2);
 }
 
 __device__ __forceinline__ int64_t addr_level(const Addr &a, int64_t j, int level)

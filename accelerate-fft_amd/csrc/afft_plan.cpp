@@ -683,8 +683,10 @@ static void *slot_ptr(Slot s, const void *in, void *out, const Plan &plan)
 }
 
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
-              hipStream_t s)
+              hipStream_t s, const int64_t *in_rows, const int64_t *out_rows)
 {
+    if ((in_rows || out_rows) && (plan.rank != 2 || plan.axes != AFFT_AXES_INNERMOST))
+        return AFFT_ERROR_INTERNAL;
     if (plan.total == 0) return AFFT_SUCCESS;
     const void *src_in = d_in;
     void *tmp = nullptr;
@@ -719,6 +721,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
+            if (l.src == Slot::kIn) a.ain.toff = in_rows;
+            if (l.dst == Slot::kOut) a.aout.toff = out_rows;
             e = l.kind == Launch::kRegCol ? launch_regcol(plan.dtype, a, l.grid, s)
                                           : launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
         } else if (l.kind == Launch::kRows) {
@@ -730,6 +734,11 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
+            if (in_rows || out_rows) {
+                // one offset table serves both sides: rows at the same offsets in and out
+                if (in_rows != out_rows || l.src != Slot::kIn || l.dst != Slot::kOut) return AFFT_ERROR_INTERNAL;
+                a.roffs = in_rows;
+            }
             e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
         } else if (l.kind == Launch::kChirp) {
             ChirpArgs a = l.ca;
@@ -740,6 +749,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
+            if (l.src == Slot::kIn) a.a.toff = in_rows;
+            if (l.dst == Slot::kOut) a.a.toff = out_rows;
             e = launch_chirp(plan.dtype, a, s);
         } else {
             BluesteinArgs a = l.ba;
@@ -750,6 +761,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
+            if (l.src == Slot::kIn) a.ain.toff = in_rows;
+            if (l.dst == Slot::kOut) a.aout.toff = out_rows;
             e = launch_bluestein(plan.dtype, a, l.block, l.grid, l.lds, s);
         }
         if (e != hipSuccess) st = AFFT_ERROR_HIP;

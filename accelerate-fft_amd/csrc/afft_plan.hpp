@@ -68,8 +68,10 @@ int device_info(int device, DeviceInfo &out);
 int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes, int device,
                const DeviceInfo &dev, hipStream_t s);
 void free_plan(Plan &plan);
+// in_rows / out_rows (optional, plans of shape (count, L) along the innermost axis only): element
+// offsets of the count rows in d_in / d_out, for the rows of one length of a ragged batch
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
-              hipStream_t s);
+              hipStream_t s, const int64_t *in_rows = nullptr, const int64_t *out_rows = nullptr);
 int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
 int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
 int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream);

@@ -11,47 +11,25 @@ namespace afft {
 
 constexpr int kRaggedBlock = 256;
 
-enum RaggedKind { kDirect = 0, kBlue = 1, kTwoPhase = 2, kBlueTwo = 3 };
+enum RaggedKind { kDirect = 0, kBlue = 1 };
 
-// one column phase of an in-workgroup four-step transform
-struct RaggedSub {
-    PassTable pt;
-    const void *tw;
-    int E;
-    int C; // columns per LDS tile
-};
-
-// one distinct length of the batch
+// one distinct length of the batch run by the persistent kernel
 struct RaggedClass {
     int kind;
     int L;          // true length
-    int E, C;       // kDirect / kBlue: elements per thread, transforms per item
+    int E, C;       // elements per thread, transforms per item
     int M;          // Bluestein padded length
     PassTable pt;   // kDirect: length L; kBlue: length M
     const void *tw;
     const void *chirp;
     const void *filt;
-    int64_t Lx;     // kTwoPhase: L; kBlueTwo: M;  Lx = s1.pt.n * s2.pt.n
-    RaggedSub s1, s2;
-    const void *hi; // W_Lx^e = hi[e >> shift] * lo[e & (2^shift - 1)]
-    const void *lo;
-    int shift;
 };
 
 struct RaggedItem {
     int cls;
     int first;  // first index into `moffs` (the transforms of the class, in class order)
-    int count;  // short transforms: transforms in the item; long ones: column tiles in the item
-    int stage;  // long transforms: the stage (= phase) of the item
-    int tile;   // long transforms: the first column tile of the item
-    int big;    // four-step Bluestein transforms: scratch slot
+    int count;  // transforms in the item
 };
-
-// stages of a long transform: A = column transforms of length L1 (twiddled) into scratch,
-// B = column transforms of length L2 out of it.  Four-step Bluestein: A1 (chirp on load),
-// B1 (filter on store), A2, B2 (chirp on store).
-constexpr int kStagesTwo = 2;
-constexpr int kStagesBlue = 4;
 
 struct RaggedArgs {
     const void *in;
@@ -60,8 +38,6 @@ struct RaggedArgs {
     const RaggedItem *items;
     const int64_t *moffs;   // element offset of every transform, in class-member order
     int nitems;
-    const int64_t *big_off; // scratch offset (elements) of every four-step Bluestein transform
-    void *scratch;
     int conj;               // Reverse / Inverse
     int inverse;            // divide every transform by its own length
 };

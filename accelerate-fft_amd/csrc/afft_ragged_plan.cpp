@@ -1,19 +1,22 @@
 // afft_ragged_plan.cpp -- plans for ragged batches (afft_ragged_* in include/afft.h).
 //
-// Planning: transforms are grouped by length into classes; each class gets a kernel path
-// (direct LDS transform, fused Bluestein, four-step, or four-step Bluestein), its FP64-generated
-// tables, and is cut into work items.  A long (four-step) transform becomes one item per column
-// tile of each stage, and stage p runs in phase p; the short items fill the phases to balance
-// their estimated cost.  Within a phase, items are ordered by decreasing cost
-// (longest-processing-time first).
+// Planning: transforms are grouped by length into classes.  A class whose transform fits one
+// workgroup of the persistent kernel (a direct LDS transform, or a fused Bluestein transform
+// whose padded length fits) gets its FP64-generated tables and is cut into work items of C
+// transforms; the items of all such classes are ordered by decreasing estimated cost
+// (longest-processing-time first) and run by one persistent launch.  Every longer class runs
+// as a plan of the batched engine on the rows of that class, gathered by their offsets: the
+// compiled row kernels, the four-step passes and the Bluestein kernels, each launch filling the
+// device with hundreds of same-length rows.  Execution is the persistent launch followed by the
+// class plans on one stream, and the graph path captures all of them as one hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
-#include <numeric>
 #include <tuple>
 #include <vector>
 
@@ -23,23 +26,24 @@
 
 namespace afft {
 
+// the rows of one long class, run by a batched plan of shape (count, L)
+struct RaggedLong {
+    Plan plan;
+    int64_t L = 0;
+    const int64_t *rows = nullptr; // device: element offsets of the class's rows
+};
+
 struct RaggedPlan {
     int dtype = 0, device = 0;
     int64_t count = 0, total = 0;
-    Plan tables;                 // owns every device allocation
+    Plan tables;                 // owns the persistent kernel's device allocations
     RaggedClass *d_classes = nullptr;
     RaggedItem *d_items = nullptr;
     int64_t *d_offsets = nullptr; // member offsets in class order
-    int64_t *d_big_off = nullptr;
-    void *d_scratch = nullptr;
-    bool has_long = false;
     int nitems = 0;
-    // phase p runs stage p of every long transform; phases are separate launches, so a stage
-    // sees every store of the one before it
-    int nphase = 0;
-    int phase_off[kStagesBlue] = {}, phase_n[kStagesBlue] = {};
-    int64_t phase_grid[kStagesBlue] = {};
+    int64_t grid = 0;
     size_t lds = 0;
+    std::vector<std::unique_ptr<RaggedLong>> longs;
     std::mutex graph_mu;
     std::map<std::tuple<int, const void *, void *>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
@@ -52,7 +56,7 @@ constexpr size_t kLdsBudget[2] = {40 * 1024, 72 * 1024};
 
 size_t esz_of(int dtype) { return dtype == AFFT_C64 ? 8 : 16; }
 
-// direct layout of length n within the ragged kernel's limits, or false
+// direct layout of length n within the persistent kernel's limits, or false
 bool ragged_layout(int64_t n, int dtype, LdsLayout &L)
 {
     if (!choose_layout(n, dtype, L, kMaxE[dtype], kRaggedBlock)) return false;
@@ -77,105 +81,39 @@ struct Builder {
     size_t budget;
     size_t lds_need = 0;
 
-    bool sub(int64_t len, RaggedSub &S)
-    {
-        LdsLayout L{};
-        if (!ragged_layout(len, rp.dtype, L)) return false;
-        const void *tw = nullptr;
-        int st = table_pass_twiddles(rp.tables, dev, s, len, L, S.pt, tw);
-        if (st) {
-            status = st;
-            return false;
-        }
-        S.tw = tw;
-        S.E = L.E;
-        const int T = (int)(len / L.E);
-        S.C = std::max(1, kRaggedBlock / T);
-        while (S.C > 1 && (size_t)S.C * lds_padded_len((int)len) * esz_of(rp.dtype) > budget) S.C /= 2;
-        lds_need = std::max(lds_need, (size_t)S.C * lds_padded_len((int)len) * esz_of(rp.dtype));
-        return true;
-    }
-
-    // four-step factors of Lx, both with a direct layout, as balanced as possible
-    bool two_phase(int64_t Lx, RaggedClass &K)
-    {
-        int64_t best1 = 0;
-        double best = 1e300;
-        for (int64_t d = 2; d < Lx && d <= (1 << 14); ++d) {
-            if (Lx % d) continue;
-            LdsLayout a{}, b{};
-            if (!ragged_layout(d, rp.dtype, a) || !ragged_layout(Lx / d, rp.dtype, b)) continue;
-            // balanced factors: measured faster than fewest-pass splits, whose long strided
-            // columns coalesce badly
-            const double sc = std::fabs(std::log((double)d) - std::log((double)(Lx / d)));
-            if (sc < best) {
-                best = sc;
-                best1 = d;
-            }
-        }
-        if (!best1) return false;
-        K.Lx = Lx;
-        if (!sub(best1, K.s1) || !sub(Lx / best1, K.s2)) return false;
-        int sh = 0;
-        while (((int64_t)1 << (2 * sh)) < Lx) ++sh;
-        const int64_t lo_cnt = (int64_t)1 << sh, hi_cnt = (Lx + lo_cnt - 1) / lo_cnt;
-        int st = table_power(rp.tables, dev, s, Lx, 1, lo_cnt, K.lo);
-        if (!st) st = table_power(rp.tables, dev, s, Lx, lo_cnt, hi_cnt, K.hi);
-        if (st) {
-            status = st;
-            return false;
-        }
-        K.shift = sh;
-        return true;
-    }
-
+    // a class of the persistent kernel, or false when the length needs a batched plan
     bool make_class(int64_t L, RaggedClass &K)
     {
         K = RaggedClass{};
         K.L = (int)L;
         LdsLayout lay{};
         const void *tw = nullptr;
-        if (is_5smooth(L) && ragged_layout(L, rp.dtype, lay)) {
+        int64_t len = L;
+        if (is_5smooth(L)) {
+            if (!ragged_layout(L, rp.dtype, lay)) return false;
             K.kind = kDirect;
-            K.E = lay.E;
-            int st = table_pass_twiddles(rp.tables, dev, s, L, lay, K.pt, tw);
+        } else {
+            const int64_t M = bluestein_length(L);
+            if (!ragged_layout(M, rp.dtype, lay)) return false;
+            K.kind = kBlue;
+            K.M = (int)M;
+            len = M;
+            int st = table_bluestein(rp.tables, dev, s, L, M, K.chirp, K.filt);
             if (st) {
                 status = st;
                 return false;
             }
-            K.tw = tw;
-            K.C = tile_count(L, lay.E, rp.dtype, budget, lay.npass);
-            if (lay.npass > 1)
-                lds_need = std::max(lds_need, (size_t)K.C * lds_padded_len((int)L) * esz_of(rp.dtype));
-            return true;
         }
-        if (is_5smooth(L)) {
-            K.kind = kTwoPhase;
-            return two_phase(L, K);
-        }
-        const int64_t M = bluestein_length(L);
-        K.M = (int)M;
-        int st = table_bluestein(rp.tables, dev, s, L, M, K.chirp, K.filt);
+        K.E = lay.E;
+        int st = table_pass_twiddles(rp.tables, dev, s, len, lay, K.pt, tw);
         if (st) {
             status = st;
             return false;
         }
-        if (ragged_layout(M, rp.dtype, lay)) {
-            K.kind = kBlue;
-            K.E = lay.E;
-            st = table_pass_twiddles(rp.tables, dev, s, M, lay, K.pt, tw);
-            if (st) {
-                status = st;
-                return false;
-            }
-            K.tw = tw;
-            K.C = tile_count(M, lay.E, rp.dtype, budget, lay.npass);
-            if (lay.npass > 1)
-                lds_need = std::max(lds_need, (size_t)K.C * lds_padded_len((int)M) * esz_of(rp.dtype));
-            return true;
-        }
-        K.kind = kBlueTwo;
-        return two_phase(M, K);
+        K.tw = tw;
+        K.C = tile_count(len, lay.E, rp.dtype, budget, lay.npass);
+        if (lay.npass > 1) lds_need = std::max(lds_need, (size_t)K.C * lds_padded_len((int)len) * esz_of(rp.dtype));
+        return true;
     }
 };
 
@@ -201,7 +139,6 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         acc += lengths[i];
     }
     rp.total = acc;
-    // classes
     std::map<int64_t, std::vector<int>> by_len;
     for (int64_t i = 0; i < rp.count; ++i)
         if (lengths[i] > 0) by_len[lengths[i]].push_back((int)i);
@@ -213,69 +150,36 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         RaggedItem it;
         double cost;
     };
-    // short transforms: one item per tile of C transforms; long transforms: one item per
-    // column tile of every stage, in the phase of that stage
-    std::vector<Tmp> shorts, phase[kStagesBlue];
-    std::vector<int64_t> big_off;
-    int64_t scratch = 0;
+    std::vector<Tmp> items;
+    std::vector<int> long_first; // first member of each long class
     for (auto &kv : by_len) {
-        RaggedClass K{};
-        if (!b.make_class(kv.first, K)) return b.status ? b.status : AFFT_ERROR_UNSUPPORTED_SIZE;
-        const int cls = (int)classes.size();
-        classes.push_back(K);
         const int first = (int)members.size();
         members.insert(members.end(), kv.second.begin(), kv.second.end());
         const int cnt = (int)kv.second.size();
-        if (K.kind == kDirect || K.kind == kBlue) {
-            const double len = (double)(K.kind == kDirect ? K.L : K.M);
-            const double per = len * std::log2(std::max(2.0, len)) * (K.kind == kBlue ? 2.5 : 1.0);
-            for (int j = 0; j < cnt; j += K.C) {
-                const int c = std::min(K.C, cnt - j);
-                shorts.push_back({{cls, first + j, c, 0, 0, 0}, per * c});
-            }
+        RaggedClass K{};
+        if (!b.make_class(kv.first, K)) {
+            if (b.status) return b.status;
+            auto lp = std::make_unique<RaggedLong>();
+            lp->L = kv.first;
+            const int64_t ext[2] = {cnt, kv.first};
+            const int st = build_plan(lp->plan, rp.dtype, 2, ext, AFFT_AXES_INNERMOST, rp.device, dev, s);
+            rp.longs.push_back(std::move(lp));
+            if (st) return st;
+            long_first.push_back(first);
             continue;
         }
-        rp.has_long = true;
-        const int nst = K.kind == kBlueTwo ? kStagesBlue : kStagesTwo;
-        rp.nphase = std::max(rp.nphase, nst);
-        const int64_t L1 = K.s1.pt.n, L2 = K.Lx / L1;
-        for (int j = 0; j < cnt; ++j) {
-            int big = 0;
-            if (K.kind == kBlueTwo) {
-                big = (int)big_off.size();
-                big_off.push_back(scratch);
-                scratch += 2 * K.Lx;
-            }
-            for (int st = 0; st < nst; ++st) {
-                const RaggedSub &S = st % 2 == 0 ? K.s1 : K.s2;
-                const int64_t cols = st % 2 == 0 ? L2 : L1;
-                const int tiles = (int)((cols + S.C - 1) / S.C);
-                const double len = (double)S.pt.n;
-                const double per = len * std::log2(std::max(2.0, len)) * S.C * 1.3;
-                for (int tl = 0; tl < tiles; ++tl) phase[st].push_back({{cls, first + j, 1, st, tl, big}, per});
-            }
+        const int cls = (int)classes.size();
+        classes.push_back(K);
+        const double len = (double)(K.kind == kDirect ? K.L : K.M);
+        const double per = len * std::log2(std::max(2.0, len)) * (K.kind == kBlue ? 2.5 : 1.0);
+        for (int j = 0; j < cnt; j += K.C) {
+            const int c = std::min(K.C, cnt - j);
+            items.push_back({{cls, first + j, c}, per * c});
         }
     }
-    rp.nphase = std::max(rp.nphase, 1);
-    // the short items go where the phases are lightest, largest first
-    std::stable_sort(shorts.begin(), shorts.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
-    double load[kStagesBlue] = {};
-    for (int p = 0; p < rp.nphase; ++p)
-        for (const Tmp &t : phase[p]) load[p] += t.cost;
-    for (const Tmp &t : shorts) {
-        int p = 0;
-        for (int q = 1; q < rp.nphase; ++q)
-            if (load[q] < load[p]) p = q;
-        phase[p].push_back(t);
-        load[p] += t.cost;
-    }
+    std::stable_sort(items.begin(), items.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
     std::vector<RaggedItem> its;
-    for (int p = 0; p < rp.nphase; ++p) {
-        std::stable_sort(phase[p].begin(), phase[p].end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
-        rp.phase_off[p] = (int)its.size();
-        rp.phase_n[p] = (int)phase[p].size();
-        for (auto &t : phase[p]) its.push_back(t.it);
-    }
+    for (auto &t : items) its.push_back(t.it);
     rp.nitems = (int)its.size();
     rp.lds = b.lds_need;
     int st = AFFT_SUCCESS;
@@ -284,32 +188,42 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
     std::vector<int64_t> moffs(members.size());
     for (size_t i = 0; i < members.size(); ++i) moffs[i] = offsets[members[i]];
     if (!st) rp.d_offsets = upload(rp, dev, moffs, s, st);
-    if (!st) rp.d_big_off = upload(rp, dev, big_off, s, st);
-    if (!st && scratch > 0) rp.d_scratch = plan_alloc(rp.tables, dev, scratch * (int64_t)esz_of(rp.dtype), st);
     if (st) return st;
+    for (size_t i = 0; i < rp.longs.size(); ++i) rp.longs[i]->rows = rp.d_offsets + long_first[i];
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
-    for (int p = 0; p < rp.nphase; ++p) rp.phase_grid[p] = std::min<int64_t>(resident, rp.phase_n[p]);
+    rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
 
+static void destroy_ragged(RaggedPlan *rp)
+{
+    for (auto &lp : rp->longs) free_plan(lp->plan);
+    free_plan(rp->tables);
+    delete rp;
+}
+
+// The persistent launch, then the batched plan of every long class.  Every transform reads and
+// writes only its own rows, so in-place execution (in == out) needs no copy.
 static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s)
 {
-    if (rp.nitems == 0) return AFFT_SUCCESS;
-    RaggedArgs a{};
-    a.in = in;
-    a.out = out;
-    a.big_off = rp.d_big_off;
-    a.classes = rp.d_classes;
-    a.moffs = rp.d_offsets;
-    a.scratch = rp.d_scratch;
-    a.conj = mode == AFFT_FORWARD ? 0 : 1;
-    a.inverse = mode == AFFT_INVERSE ? 1 : 0;
-    for (int p = 0; p < rp.nphase; ++p) {
-        if (rp.phase_n[p] == 0) continue;
-        a.items = rp.d_items + rp.phase_off[p];
-        a.nitems = rp.phase_n[p];
-        if (launch_ragged(rp.dtype, a, rp.phase_grid[p], rp.lds, s) != hipSuccess) return AFFT_ERROR_HIP;
+    if (rp.nitems > 0) {
+        RaggedArgs a{};
+        a.in = in;
+        a.out = out;
+        a.nitems = rp.nitems;
+        a.classes = rp.d_classes;
+        a.items = rp.d_items;
+        a.moffs = rp.d_offsets;
+        a.conj = mode == AFFT_FORWARD ? 0 : 1;
+        a.inverse = mode == AFFT_INVERSE ? 1 : 0;
+        if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, s) != hipSuccess) return AFFT_ERROR_HIP;
+    }
+    const int direction = mode == AFFT_FORWARD ? -1 : +1;
+    for (const auto &lp : rp.longs) {
+        const double scale = mode == AFFT_INVERSE ? 1.0 / (double)lp->L : 1.0;
+        const int st = exec_plan(lp->plan, direction, in, out, scale, s, lp->rows, lp->rows);
+        if (st) return st;
     }
     return AFFT_SUCCESS;
 }
@@ -348,8 +262,7 @@ extern "C" int afft_ragged_plan_create(afft_ragged_plan **plan, int dtype, int64
     (void)hipStreamDestroy(s);
     if (old >= 0) (void)hipSetDevice(old);
     if (st) {
-        free_plan(rp->tables);
-        delete rp;
+        destroy_ragged(rp);
         return st;
     }
     *plan = reinterpret_cast<afft_ragged_plan *>(rp);
@@ -367,30 +280,15 @@ extern "C" int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const vo
     int old = -1;
     (void)hipGetDevice(&old);
     if (old != rp->device && hipSetDevice(rp->device) != hipSuccess) return AFFT_ERROR_HIP;
-    hipStream_t s = (hipStream_t)stream;
-    const void *src = d_in;
-    void *tmp = nullptr;
-    // stage A of a long transform writes transposed columns into the output: in place, the
-    // input is read from a private copy
-    if (d_in == d_out && rp->has_long) {
-        const size_t bytes = (size_t)rp->total * (rp->dtype == AFFT_C64 ? 8 : 16);
-        if (hipMallocAsync(&tmp, bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
-        if (hipMemcpyAsync(tmp, d_in, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-            (void)hipFreeAsync(tmp, s);
-            return AFFT_ERROR_HIP;
-        }
-        src = tmp;
-    }
-    const int st = ragged_launch(*rp, mode, src, d_out, s);
-    if (tmp) (void)hipFreeAsync(tmp, s);
+    const int st = ragged_launch(*rp, mode, d_in, d_out, (hipStream_t)stream);
     if (old >= 0 && old != rp->device) (void)hipSetDevice(old);
     return st;
 }
 
-// Capture the phase launches once per (mode, in, out) on a private stream and replay the
-// instantiated graph on the caller's stream (capture is not allowed on the legacy default
-// stream).  The graph holds only kernel nodes: the kernel keeps no device-side state, so there
-// is no memset node to replay.
+// Capture the launches once per (mode, in, out) on a private stream and replay the instantiated
+// graph on the caller's stream (capture is not allowed on the legacy default stream).  The graph
+// holds only kernel nodes: no kernel keeps device-side state, so there is no memset node to
+// replay.
 extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
                                       void *stream)
 {
@@ -414,7 +312,6 @@ extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const vo
             hipStream_t cap;
             if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) st = AFFT_ERROR_HIP;
             hipGraph_t g = nullptr;
-            if (!st && d_in == d_out && rp->has_long) st = AFFT_ERROR_INVALID_VALUE; // needs a copy: use afft_ragged_exec
             if (!st && hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) st = AFFT_ERROR_HIP;
             if (!st) {
                 const int lst = ragged_launch(*rp, mode, d_in, d_out, cap);
@@ -443,9 +340,8 @@ extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *plan)
         (void)hipGraphExecDestroy(kv.second.second);
         (void)hipGraphDestroy(kv.second.first);
     }
-    free_plan(rp->tables);
+    destroy_ragged(rp);
     if (old >= 0) (void)hipSetDevice(old);
-    delete rp;
     return AFFT_SUCCESS;
 }
 

@@ -69,6 +69,13 @@ def test_ragged_in_place_and_empty_rows(gpu):
     out = plan.execute(Mode.Forward, y, out=y)
     assert out.data_ptr() == y.data_ptr()
     check_rows(y, x, lengths, Mode.Forward, torch.complex64)
+    # every transform reads and writes only its own rows: in place through the graph too
+    z = x.clone()
+    for _ in range(2):
+        z.copy_(x)
+        plan.execute(Mode.Inverse, z, out=z, graph=True)
+        torch.cuda.synchronize()
+        check_rows(z, x, lengths, Mode.Inverse, torch.complex64)
     plan.close()
 
 
