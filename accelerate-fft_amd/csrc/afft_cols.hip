@@ -1,4 +1,5 @@
-// afft_cols.hip -- register-resident transforms for the passes of a strided axis.
+// afft_cols.hip -- register-resident transforms for the passes of a strided axis, and the
+// batched transpose that ends a six-step transform.
 //
 //   afft_regcol<Real, R>   one thread owns one whole length-R transform of a strided batch:
 //                          R loads, an in-register DFT_R (dft_split for R = 32, 64), the optional
@@ -96,6 +97,64 @@ hipError_t launch_regcol(int dtype, const FftArgs &a, int64_t grid, hipStream_t 
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffff) return hipErrorInvalidValue;
     return dtype == AFFT_C64 ? launch_regcol_t<float>(a, grid, s) : launch_regcol_t<double>(a, grid, s);
+}
+
+
+// ----------------------------------------------------------------------------------------------
+// Batched transpose, the last pass of a six-step transform: matrix o (rows x cols, row-major)
+// is written transposed (cols x rows) at out + (toff ? toff[o] : o * rows * cols).  A square
+// tile goes through LDS so that both the loads (along a row of the input) and the stores (along
+// a row of the output) are contiguous runs of TILE elements; the LDS row pitch TILE + 1 keeps
+// the column-wise LDS reads off a single bank.
+
+template <typename Real, int TILE>
+__global__ void __launch_bounds__(256) afft_transpose(const TransposeArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    constexpr int RS = 256 / TILE; // tile rows per sweep
+    __shared__ V tile[TILE][TILE + 1];
+    const int64_t b = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t ntx = (a.cols + TILE - 1) / TILE, nty = (a.rows + TILE - 1) / TILE;
+    const int64_t bx = b % ntx, q = b / ntx;
+    const int64_t by = q % nty, o = q / nty;
+    if (o >= a.nbatch) return; // whole workgroup: no barrier is skipped by part of it
+    const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+    const V *in = reinterpret_cast<const V *>(a.in) + o * a.rows * a.cols;
+    V *out = reinterpret_cast<V *>(a.out) + (a.toff ? a.toff[o] : o * a.rows * a.cols);
+#pragma unroll
+    for (int i = 0; i < TILE / RS; ++i) {
+        const int64_t r = by * TILE + ty + i * RS, c = bx * TILE + tx;
+        if (r < a.rows && c < a.cols) tile[ty + i * RS][tx] = in[r * a.cols + c];
+    }
+    __syncthreads();
+    const Real sc = (Real)a.scale;
+#pragma unroll
+    for (int i = 0; i < TILE / RS; ++i) {
+        const int64_t c = bx * TILE + ty + i * RS, r = by * TILE + tx;
+        if (r < a.rows && c < a.cols) {
+            V v = tile[tx][ty + i * RS];
+            if (a.conj_out) v = c_conj(v);
+            c_store(out + c * a.rows + r, c_scale(v, sc), (bool)a.nt_out);
+        }
+    }
+}
+
+int64_t transpose_grid(int dtype, int64_t rows, int64_t cols, int64_t nbatch)
+{
+    const int64_t T = dtype == AFFT_C64 ? 64 : 32;
+    return ((cols + T - 1) / T) * ((rows + T - 1) / T) * nbatch;
+}
+
+hipError_t launch_transpose(int dtype, const TransposeArgs &a, hipStream_t s)
+{
+    const int64_t grid = transpose_grid(dtype, a.rows, a.cols, a.nbatch);
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    if (dtype == AFFT_C64)
+        hipLaunchKernelGGL((afft_transpose<float, 64>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((afft_transpose<double, 32>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 } // namespace afft

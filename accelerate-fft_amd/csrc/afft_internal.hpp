@@ -130,6 +130,20 @@ int row_rows_per_tile(int dtype, int n, int variant);
 int row_occupancy(int dtype, int n, int variant);
 hipError_t launch_rows(int dtype, int n, int variant, const RowArgs &a, int64_t grid, hipStream_t s);
 
+// Batched transpose (afft_cols.hip): matrix o of nbatch (rows x cols, row-major, contiguous in
+// `in`) stored transposed at out + (toff ? toff[o] : o * rows * cols), conjugated and scaled.
+struct TransposeArgs {
+    const void *in;
+    void *out;
+    int64_t rows, cols, nbatch;
+    const int64_t *toff;
+    int conj_out;
+    double scale;
+    int nt_out;
+    int xcd_remap;
+};
+hipError_t launch_transpose(int dtype, const TransposeArgs &a, hipStream_t s);
+
 // Launchers (afft_kernels.hip).  They return a hipError_t.
 // register-resident strided transforms (afft_cols.hip): one thread per transform of length
 // a.E, kRegColBlock threads per workgroup, no LDS

@@ -480,6 +480,46 @@ struct Builder {
         plan.launches.push_back(l);
     }
 
+    // the variant being timed by the autotuner, else an explicit AFFT_ROW_VARIANT, else the
+    // autotuned choice, else the default layout
+    int pick_row_variant(int64_t n)
+    {
+        int variant = forced_row_variant();
+        if (variant < 0) {
+            if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
+            else variant = tuned_row_variant(plan.dtype, n, plan.device);
+        }
+        if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
+        return variant;
+    }
+
+    void add_transpose(int64_t nbatch, int64_t rows, int64_t cols, Slot src, Slot dst)
+    {
+        Launch l{};
+        l.kind = Launch::kTranspose;
+        l.src = src;
+        l.dst = dst;
+        l.ta.rows = rows;
+        l.ta.cols = cols;
+        l.ta.nbatch = nbatch;
+        l.ta.scale = 1.0;
+        l.ta.xcd_remap = !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
+        plan.launches.push_back(l);
+    }
+
+    // Six-step factors of a contiguous row of length n (n = n1 * n2): a register-resident column
+    // length n1 whose rows n2 have a compiled row kernel; 0 if none (the balanced four-step,
+    // whose LDS passes beat a six-step with an LDS column pass).
+    int64_t six_step_n1(int64_t n)
+    {
+        if (getenv_flag("AFFT_NO_SIX_STEP")) return 0;
+        for (int64_t c : {64, 32, 16})
+            if (n % c == 0 && regcol_supported(plan.dtype, c) && n / c <= (1 << 30) &&
+                row_variants(plan.dtype, (int)(n / c)) > 0)
+                return c;
+        return 0;
+    }
+
     // Transform along one axis: extent n, element stride s_ (product of the inner extents),
     // o outer blocks; reads slot src, writes slot dst.
     enum class AxisKind { kRows, kSingle, kBluestein, kSplit };
@@ -495,6 +535,61 @@ struct Builder {
             return AxisKind::kSingle;
         if (!is_5smooth(n)) return AxisKind::kBluestein;
         return AxisKind::kSplit;
+    }
+
+    // two-level table for W_n^e: lo[e & (2^sh - 1)], hi[e >> sh]
+    bool power_tables(int64_t n, const void *&hi, const void *&lo, int &sh)
+    {
+        sh = 0;
+        while (((int64_t)1 << (2 * sh)) < n) ++sh;
+        const int64_t lo_cnt = (int64_t)1 << sh;
+        const int64_t hi_cnt = (n + lo_cnt - 1) / lo_cnt;
+        void *l = alloc(lo_cnt * (int64_t)esz());
+        void *h = alloc(hi_cnt * (int64_t)esz());
+        if (status) return false;
+        check(gen_power_table(plan.dtype, l, n, 1, lo_cnt, s));
+        check(gen_power_table(plan.dtype, h, n, lo_cnt, hi_cnt, s));
+        lo = l;
+        hi = h;
+        return status == AFFT_SUCCESS;
+    }
+
+    // Six-step transform of o contiguous rows of length n = n1 * n2 (element i = i1*n2 + i2,
+    // output k = k1 + n1*k2), three passes that each read and write whole lines:
+    //   A  columns: length n1 over i1 (stride n2), twiddled by W_n^{i2 k1}, in place (src -> cur)
+    //   B  rows: length n2 over i2 for every (o, k1), in place on cur (compiled row kernel)
+    //   C  transpose of every n1 x n2 matrix into other(cur), putting X[k1 + n1*k2] in place
+    void add_six_step(int64_t n, int64_t n1, int64_t o, Slot src, Slot cur)
+    {
+        const int64_t n2 = n / n1;
+        const void *hi = nullptr, *lo = nullptr;
+        int sh = 0;
+        if (!power_tables(n, hi, lo, sh)) return;
+        const Slot next = other(cur);
+        int64_t &room = next == Slot::kWork ? plan.work_elems
+                        : next == Slot::kAux2 ? plan.aux2_elems : plan.aux_elems;
+        if (next != Slot::kOut) room = std::max(room, o * n);
+        Addr ia{};
+        ia.cnt0 = 1;
+        ia.cnt1 = n2;
+        ia.st0 = 1;
+        ia.st1 = 1;
+        ia.st2 = n;
+        ia.es = n2;
+        add_stockham(n1, o * n2, ia, ia, true, src, cur, n, 1, hi, lo, sh);
+        if (status) return;
+        if (row_variants(plan.dtype, (int)n2) > 0 && !getenv_flag("AFFT_NO_ROWS")) {
+            add_rows(n2, o * n1, cur, cur, pick_row_variant(n2));
+        } else {
+            Addr ra{};
+            ra.cnt0 = 1;
+            ra.cnt1 = 1;
+            ra.st2 = n2;
+            ra.es = 1;
+            add_stockham(n2, o * n1, ra, ra, false, cur, cur);
+        }
+        if (status) return;
+        add_transpose(o, n1, n2, cur, next);
     }
 
     // Transform one axis, reading slot src.  Single-pass kinds run in place or out of place and
@@ -514,21 +609,16 @@ struct Builder {
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
         switch (axis_kind(n, s_)) {
-        case AxisKind::kRows: {
-            // the variant being timed by the autotuner, else an explicit AFFT_ROW_VARIANT, else
-            // the autotuned choice, else the default layout
-            int variant = forced_row_variant();
-            if (variant < 0) {
-                if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
-                else variant = tuned_row_variant(plan.dtype, n, plan.device);
-            }
-            if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
-            add_rows(n, ntrans, src, cur, variant);
-            return cur;
-        }
+        case AxisKind::kRows: add_rows(n, ntrans, src, cur, pick_row_variant(n)); return cur;
         case AxisKind::kSingle: add_stockham(n, ntrans, a, a, strided, src, cur); return cur;
         case AxisKind::kBluestein: add_bluestein(n, ntrans, a, a, strided, src, cur); return cur;
         case AxisKind::kSplit: break;
+        }
+        if (s_ == 1) {
+            if (const int64_t n1 = six_step_n1(n)) {
+                add_six_step(n, n1, o, src, cur);
+                return other(cur);
+            }
         }
         // four-step: n = n1 * n2, element i = i1*n2 + i2, output k = k1 + n1*k2
         int64_t n1 = 0, n2 = 0;
@@ -734,12 +824,22 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.conj_in = first ? conj : 0;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
-            if (in_rows || out_rows) {
+            if ((in_rows || out_rows) && (l.src == Slot::kIn || l.dst == Slot::kOut)) {
                 // one offset table serves both sides: rows at the same offsets in and out
                 if (in_rows != out_rows || l.src != Slot::kIn || l.dst != Slot::kOut) return AFFT_ERROR_INTERNAL;
                 a.roffs = in_rows;
             }
             e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
+        } else if (l.kind == Launch::kTranspose) {
+            TransposeArgs a = l.ta;
+            a.in = slot_ptr(l.src, src_in, d_out, plan);
+            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
+            a.conj_out = last ? conj : 0;
+            a.scale = last ? scale : 1.0;
+            if (l.src == Slot::kIn || first) return AFFT_ERROR_INTERNAL;
+            if (l.dst == Slot::kOut) a.toff = out_rows;
+            e = launch_transpose(plan.dtype, a, s);
         } else if (l.kind == Launch::kChirp) {
             ChirpArgs a = l.ca;
             a.in = slot_ptr(l.src, src_in, d_out, plan);

@@ -28,12 +28,13 @@ struct DeviceInfo {
 enum class Slot { kIn, kOut, kWork, kAux, kAux2 };
 
 struct Launch {
-    enum Kind { kStockham, kBluestein, kRows, kChirp, kRegCol } kind;
+    enum Kind { kStockham, kBluestein, kRows, kChirp, kRegCol, kTranspose } kind;
     Slot src, dst;
     FftArgs fa;
     BluesteinArgs ba;
     RowArgs ra;
     ChirpArgs ca;
+    TransposeArgs ta;
     int row_n = 0, row_variant = 0;
     int block;
     int64_t grid;

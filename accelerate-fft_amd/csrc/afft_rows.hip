@@ -290,6 +290,13 @@ const RowEntry kRows[] = {
     AFFT_ROW(float, 0, 10240, 20, true, 2, true, 2, 4, 4, 4, 4, 4, 5),
     AFFT_ROW(float, 0, 12288, 24, true, 2, true, 8, 8, 8, 8, 3),
     AFFT_ROW(float, 0, 12288, 24, false, 2, true, 8, 8, 8, 8, 3),
+    // the row lengths of six-step transforms n = 64 * n2
+    AFFT_ROW(float, 0, 160, 20, true, 2, true, 2, 4, 4, 5),
+    AFFT_ROW(float, 0, 320, 20, true, 2, true, 4, 4, 4, 5),
+    AFFT_ROW(float, 0, 384, 24, true, 2, true, 4, 4, 8, 3),
+    AFFT_ROW(float, 0, 640, 20, true, 2, true, 4, 4, 2, 4, 5),
+    AFFT_ROW(float, 0, 768, 24, true, 2, true, 4, 8, 8, 3),
+    AFFT_ROW(float, 0, 1536, 24, true, 2, true, 8, 8, 8, 3),
     AFFT_ROW(double, 1, 256, 8, true, 2, false, 8, 8, 4),
     AFFT_ROW(double, 1, 512, 8, true, 2, false, 8, 8, 8),
     AFFT_ROW(double, 1, 1024, 8, true, 2, false, 8, 8, 8, 2),

@@ -13,7 +13,8 @@ from conftest import tolerance
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [(torch.complex64, n) for n in (256, 512, 1024, 2048, 4096, 8192, 5120, 6144, 10240, 12288)] + \
+LENGTHS = [(torch.complex64, n) for n in (256, 512, 1024, 2048, 4096, 8192, 5120, 6144, 10240, 12288,
+                                          160, 320, 384, 640, 768, 1536)] + \
           [(torch.complex128, n) for n in (256, 512, 1024, 2048, 4096)]
 
 
