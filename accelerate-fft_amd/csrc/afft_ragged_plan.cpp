@@ -31,7 +31,11 @@ struct RaggedLong {
     Plan plan;
     int64_t L = 0;
     const int64_t *rows = nullptr; // device: element offsets of the class's rows
+    bool small = false;            // too few elements to fill the device: runs on the side stream
 };
+
+// a long class below this many elements leaves most CUs idle and is latency-bound
+constexpr int64_t kSmallLongElems = 1 << 20;
 
 struct RaggedPlan {
     int dtype = 0, device = 0;
@@ -44,6 +48,12 @@ struct RaggedPlan {
     int64_t grid = 0;
     size_t lds = 0;
     std::vector<std::unique_ptr<RaggedLong>> longs;
+    // Latency-bound work (the persistent launch, long classes too small to fill the device) runs
+    // on a side stream forked from and joined back to the caller's stream, beside the
+    // bandwidth-bound long classes; launch_mu makes each fork/join pair atomic across threads.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    mutable std::mutex launch_mu;
     std::mutex graph_mu;
     std::map<std::tuple<int, const void *, void *>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
@@ -161,6 +171,7 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             if (b.status) return b.status;
             auto lp = std::make_unique<RaggedLong>();
             lp->L = kv.first;
+            lp->small = (int64_t)cnt * kv.first < kSmallLongElems;
             const int64_t ext[2] = {cnt, kv.first};
             const int st = build_plan(lp->plan, rp.dtype, 2, ext, AFFT_AXES_INNERMOST, rp.device, dev, s);
             rp.longs.push_back(std::move(lp));
@@ -192,21 +203,41 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
     for (size_t i = 0; i < rp.longs.size(); ++i) rp.longs[i]->rows = rp.d_offsets + long_first[i];
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
     rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
+    if (hipStreamCreateWithFlags(&rp.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&rp.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&rp.join, hipEventDisableTiming) != hipSuccess)
+        return AFFT_ERROR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
 
 static void destroy_ragged(RaggedPlan *rp)
 {
+    if (rp->side) (void)hipStreamSynchronize(rp->side);
     for (auto &lp : rp->longs) free_plan(lp->plan);
     free_plan(rp->tables);
+    if (rp->fork) (void)hipEventDestroy(rp->fork);
+    if (rp->join) (void)hipEventDestroy(rp->join);
+    if (rp->side) (void)hipStreamDestroy(rp->side);
     delete rp;
 }
 
-// The persistent launch, then the batched plan of every long class.  Every transform reads and
-// writes only its own rows, so in-place execution (in == out) needs no copy.
+static int run_long(const RaggedLong &lp, int mode, const void *in, void *out, hipStream_t s)
+{
+    const int direction = mode == AFFT_FORWARD ? -1 : +1;
+    const double scale = mode == AFFT_INVERSE ? 1.0 / (double)lp.L : 1.0;
+    return exec_plan(lp.plan, direction, in, out, scale, s, lp.rows, lp.rows);
+}
+
+// The persistent launch and the small long classes on the side stream, the other long classes on
+// s, joined back into s.  Every transform reads and writes only its own rows, so the branches
+// touch disjoint memory and in-place execution (in == out) needs no copy.
 static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s)
 {
+    std::lock_guard<std::mutex> lk(rp.launch_mu);
+    if (hipEventRecord(rp.fork, s) != hipSuccess || hipStreamWaitEvent(rp.side, rp.fork, 0) != hipSuccess)
+        return AFFT_ERROR_HIP;
+    int st = AFFT_SUCCESS;
     if (rp.nitems > 0) {
         RaggedArgs a{};
         a.in = in;
@@ -217,15 +248,16 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
         a.moffs = rp.d_offsets;
         a.conj = mode == AFFT_FORWARD ? 0 : 1;
         a.inverse = mode == AFFT_INVERSE ? 1 : 0;
-        if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, s) != hipSuccess) return AFFT_ERROR_HIP;
+        if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, rp.side) != hipSuccess) st = AFFT_ERROR_HIP;
     }
-    const int direction = mode == AFFT_FORWARD ? -1 : +1;
-    for (const auto &lp : rp.longs) {
-        const double scale = mode == AFFT_INVERSE ? 1.0 / (double)lp->L : 1.0;
-        const int st = exec_plan(lp->plan, direction, in, out, scale, s, lp->rows, lp->rows);
-        if (st) return st;
-    }
-    return AFFT_SUCCESS;
+    for (const auto &lp : rp.longs)
+        if (!st && lp->small) st = run_long(*lp, mode, in, out, rp.side);
+    for (const auto &lp : rp.longs)
+        if (!st && !lp->small) st = run_long(*lp, mode, in, out, s);
+    // join even after an error, so that the side stream never runs ahead of the caller's
+    if (hipEventRecord(rp.join, rp.side) != hipSuccess || hipStreamWaitEvent(s, rp.join, 0) != hipSuccess)
+        st = st ? st : AFFT_ERROR_HIP;
+    return st;
 }
 
 } // namespace afft
