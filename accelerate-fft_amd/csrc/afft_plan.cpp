@@ -216,7 +216,6 @@ struct Builder {
     static int strided_cols_target(int T)
     {
         int C = 16;
-        if (const char *v = std::getenv("AFFT_STRIDED_C")) return std::max(1, std::atoi(v));
         while (C < 64 && 2 * C * T <= 256) C *= 2;
         return C;
     }

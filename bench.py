@@ -120,8 +120,10 @@ def cpu_baseline(rows, n, min_seconds=12.0, max_reps=400):
 
 
 def ragged_extras(afft, Mode, torch, dev, args):
-    """10,000 Complex Float transforms of mixed lengths: one persistent launch (eager and as a
-    replayed hipGraph) against one plain call per transform and one batched call per length."""
+    """10,000 Complex Float transforms of mixed lengths through one ragged plan (the lengths that fit
+    one workgroup in one persistent launch, the longer ones as batched plans over their gathered
+    rows; eager and as a replayed hipGraph) against one plain call per transform and one batched
+    call per length."""
     import ctypes
     lengths = ragged_lengths()
     total = sum(lengths)

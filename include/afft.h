@@ -151,13 +151,17 @@ int afft_fft_vjp(int op, int mode, int dtype, int rank, const int64_t *extents, 
                  void *d_gx, void *stream);
 
 /* Ragged batch: `count` independent 1D transforms of the given lengths, stored back to back
- * (row i starts at element sum_{j<i} lengths[j]).  One persistent kernel drains a device-side
- * work queue; large rows and Bluestein rows are handled inside the same launch. */
+ * (row i starts at element sum_{j<i} lengths[j]).  The lengths that fit one workgroup run in one
+ * persistent kernel that drains a device-side work list; every longer length runs as a batched
+ * plan over its rows, gathered by offset, on the same stream (latency-bound parts on a forked and
+ * rejoined side stream).  In place (d_in == d_out) is allowed.  Extension: the reference has no
+ * ragged entry point (FFT.hs:63-84 batches only equal rows). */
 int afft_ragged_plan_create(afft_ragged_plan **plan, int dtype, int64_t count,
                             const int64_t *lengths, int device);
 int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
                      void *stream);
-/* The same work captured once into a hipGraph per (mode, d_in, d_out) and replayed. */
+/* The same work captured once into a hipGraph per (mode, d_in, d_out) and replayed (kernel nodes
+ * only, no memset). */
 int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
                            void *stream);
 int afft_ragged_plan_destroy(afft_ragged_plan *plan);
