@@ -22,6 +22,8 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
 
   --profile-headline runs only the headline plan (and a plain device copy of the same array,
              the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
+  --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral) for
+             warmup + steps calls, for profiling.
 """
 import argparse
 import json
@@ -209,6 +211,44 @@ def spectral_extras(afft, Mode, torch, dev, args):
             "spectral_conv_speedup_fwd_vjp": round(tp / tf, 2)}
 
 
+def _prof_fft2d(afft, Mode, torch, dev):
+    x = normal_c((4096, 4096), torch.complex64, 3, dev)
+    return lambda: afft.fft2D(Mode.Forward, x)
+
+
+def _prof_fft3d(afft, Mode, torch, dev):
+    x = normal_c((256, 256, 256), torch.complex128, 3, dev)
+    return lambda: afft.fft3D(Mode.Forward, x)
+
+
+def _prof_four_step(afft, Mode, torch, dev):
+    x = uniform_c((1, 1 << 22), torch.complex64, 2, dev)
+    return lambda: afft.fft(Mode.Forward, x)
+
+
+def _prof_ragged(afft, Mode, torch, dev):
+    lengths = ragged_lengths()
+    x = uniform_c((sum(lengths),), torch.complex64, 4, dev)
+    out = torch.empty_like(x)
+    plan = afft.RaggedPlan(lengths, torch.complex64)
+    return lambda: plan.execute(Mode.Forward, x, out=out, graph=True)
+
+
+def _prof_spectral(afft, Mode, torch, dev):
+    x = normal_c((64, 512, 512), torch.complex64, 5, dev)
+    H = normal_c((512, 512), torch.complex64, 6, dev)
+    t = normal_c((64, 512, 512), torch.complex64, 7, dev)
+
+    def call():
+        y = afft.spectral_conv(x, H)
+        afft.spectral_conv_vjp(x, H, 2 * (y - t))
+    return call
+
+
+PROFILE_WORKLOADS = {"fft2d": _prof_fft2d, "fft3d": _prof_fft3d, "four_step": _prof_four_step,
+                     "ragged": _prof_ragged, "spectral": _prof_spectral}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +257,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--profile-headline", action="store_true")
+    ap.add_argument("--profile-workload", choices=sorted(PROFILE_WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -228,6 +269,14 @@ def main():
     from afft import Mode
 
     dev = torch.device("cuda", 0)
+    if args.profile_workload:
+        call = PROFILE_WORKLOADS[args.profile_workload](afft, Mode, torch, dev)
+        for _ in range(args.warmup + args.steps):
+            call()
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_workload": args.profile_workload, "steps": args.steps}), flush=True)
+        return
+
     rows, n = 16384, 4096
     x = uniform_c((rows, n), torch.complex64, 2, dev)
     out = torch.empty_like(x)
