@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 constexpr int kVjpBlock = 256;
 
 template <typename Real, int E>
-__global__ void __launch_bounds__(kVjpBlock) afft_colconv_vjp(const ColConvArgs a)
+__global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1, 8))) afft_colconv_vjp(const ColConvArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
@@ -96,26 +96,31 @@ __global__ void __launch_bounds__(kVjpBlock) afft_colconv_vjp(const ColConvArgs 
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = c_make<V>(0, 0);
     for (int64_t b = b0; b < b1; ++b) {
+        // the thread index passes through an empty asm once per image, so the LDS and element
+        // addresses are recomputed here instead of all being hoisted out of the loop, where they
+        // would hold most of the register file and leave one wave per SIMD
+        int tl = t;
+        __asm__ volatile("" : "+v"(tl));
         const V *x = reinterpret_cast<const V *>(a.other) + b * a.h * a.w + j;
         V *gy = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
         V X[E], G[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            X[i] = active ? x[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
-            G[i] = active ? gy[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+            X[i] = active ? x[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
+            G[i] = active ? gy[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
         }
-        fft_core<E>(X, lds + c, C, a.pt, tw, t, T);
-        fft_core<E>(G, lds + c, C, a.pt, tw, t, T);
+        fft_core<E>(X, lds + c, C, a.pt, tw, tl, T);
+        fft_core<E>(G, lds + c, C, a.pt, tw, tl, T);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             acc[i] = c_add(acc[i], c_mulc(G[i], X[i])); // conj(X) G
-            const V h = active ? Hc[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
+            const V h = active ? Hc[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
             G[i] = c_conj(c_mulc(G[i], h));             // conj(conj(H) G)
         }
-        fft_core<E>(G, lds + c, C, a.pt, tw, t, T);
+        fft_core<E>(G, lds + c, C, a.pt, tw, tl, T);
         if (active) {
 #pragma unroll
-            for (int i = 0; i < E; ++i) gy[(int64_t)(t + i * T) * a.w] = c_conj(G[i]);
+            for (int i = 0; i < E; ++i) gy[(int64_t)(tl + i * T) * a.w] = c_conj(G[i]);
         }
     }
     if (active) {
