@@ -443,24 +443,37 @@ __device__ __forceinline__ void scatter_dispatch(int R, const V (&reg)[E], V *ld
     }
 }
 
+// Ordering of an LDS exchange private to one wavefront (all threads of the transform in one
+// wavefront): a wavefront's LDS instructions execute in order, so wavefront-scope ordering is
+// enough and no workgroup barrier is needed.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // The whole forward transform of the rows held in registers (natural layout in, natural layout
 // out).  All threads of the workgroup must call it (it contains barriers); threads that own no
 // transform (tail of the batch) still take part.  `lds_base`/`es` address this thread's
 // transform in LDS; `tw` holds the pass twiddles (pass p at tw + pt.tw_off[p]).
 // `on` = false for threads of the workgroup that own no transform slot: they take part in the
 // barriers but never touch LDS.
+// `wave` (uniform) = every transform's threads lie in one wavefront and its LDS slot is private
+// to it: the exchanges then synchronise at wavefront scope instead of with workgroup barriers.
 template <int E, typename V>
 __device__ __forceinline__ void fft_core(V (&reg)[E], V *lds_base, int es, const PassTable &pt,
-                                         const V *__restrict__ tw, int t, int T, bool on = true)
+                                         const V *__restrict__ tw, int t, int T, bool on = true,
+                                         bool wave = false)
 {
     int Ns = 1;
     for (int p = 0; p < pt.npass; ++p) {
         const int R = pt.radix[p];
         pass_dispatch<E>(R, reg, tw + pt.tw_off[p], Ns, t, T);
         if (p + 1 < pt.npass) {
-            __syncthreads(); // everyone is done reading the previous exchange
+            if (wave) wave_sync(); else __syncthreads(); // the previous exchange has been read
             if (on) scatter_dispatch<E>(R, reg, lds_base, es, Ns, t, T);
-            __syncthreads();
+            if (wave) wave_sync(); else __syncthreads();
             if (on) natural_read<E>(reg, lds_base, es, t, T);
         }
         Ns *= R;

@@ -38,7 +38,9 @@ __device__ __forceinline__ void run_direct(const RaggedArgs &a, const RaggedClas
         reg[i] = a.conj ? c_conj(v) : v;
     }
     V *slot = lds + (int64_t)(c < K.C ? c : 0) * lds_padded_len(L);
-    fft_core<E>(reg, slot, 1, K.pt, reinterpret_cast<const V *>(K.tw), t, T, c < K.C);
+    // transform c's threads are [c*T, c*T + T): within one wavefront when T divides 64
+    const bool wave = T <= 64 && (64 % T) == 0;
+    fft_core<E>(reg, slot, 1, K.pt, reinterpret_cast<const V *>(K.tw), t, T, c < K.C, wave);
     const Real sc = a.inverse ? (Real)(1.0 / (double)L) : (Real)1;
     if (active) {
 #pragma unroll
@@ -77,10 +79,13 @@ __device__ __forceinline__ void run_blue(const RaggedArgs &a, const RaggedClass 
     }
     V *slot = lds + (int64_t)(c < K.C ? c : 0) * lds_padded_len(M);
     const bool on = c < K.C;
-    fft_core<E>(reg, slot, 1, K.pt, tw, t, T, on);
+    const bool wave = T <= 64 && (64 % T) == 0;
+    fft_core<E>(reg, slot, 1, K.pt, tw, t, T, on, wave);
 #pragma unroll
     for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], on ? filt[t + i * T] : c_make<V>(0, 0)));
-    fft_core<E>(reg, slot, 1, K.pt, tw, t, T, on);
+    // the second transform's first exchange writes the slot the first one read last: in order
+    // within the wavefront, or behind fft_core's leading barrier
+    fft_core<E>(reg, slot, 1, K.pt, tw, t, T, on, wave);
     const Real sc = a.inverse ? (Real)(1.0 / (double)n) : (Real)1;
     if (active) {
 #pragma unroll

@@ -39,6 +39,16 @@ __device__ __forceinline__ void lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Synchronisation of an LDS exchange among the T threads of one row.  When they all sit in one
+// wavefront (T divides 64; a row's threads are contiguous), the exchange is private to that
+// wavefront, whose LDS instructions execute in order: ordering the accesses at wavefront scope
+// is enough, and no workgroup barrier stalls the other rows of the workgroup.
+template <int T> __device__ __forceinline__ void row_sync()
+{
+    if constexpr (T <= 64 && 64 % T == 0) wave_sync();
+    else lds_sync();
+}
+
 template <int... Rs> struct Radices {
     static constexpr int NP = sizeof...(Rs);
     static constexpr int R[NP] = {Rs...};
@@ -223,7 +233,7 @@ afft_rows(const RowArgs a)
                 for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
             });
             if constexpr (p + 1 < NP) {
-                lds_sync(); // the previous exchange has been read by everyone
+                row_sync<T>(); // the previous exchange has been read by every thread of the row
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     const int b = tl + j * T;
@@ -232,7 +242,7 @@ afft_rows(const RowArgs a)
 #pragma unroll
                     for (int r = 0; r < R; ++r) slot[lds_pad(dst + r * Ns)] = reg[j + r * B];
                 }
-                lds_sync();
+                row_sync<T>();
 #pragma unroll
                 for (int i = 0; i < E; ++i) reg[i] = slot[lds_pad(tl + i * T)];
             }
