@@ -123,6 +123,23 @@ def test_long_strided_axes(gpu, dtype, shape):
     check(y, O.fft_axes(xr, tuple(range(len(shape))), -1), dtype, n, f"{shape} in place")
 
 
+def test_two_split_axes(gpu):
+    """Complex Double 2048 x 16384: the rows split six-step (32 x 512) and the 2048-long columns
+    split four-step (64-byte tiles otherwise), so the plan's first pass writes the output, the
+    row transpose moves the data to the work buffer and the column pass B brings it back."""
+    shape = (2048, 16384)
+    x = O.normal(21, shape)
+    xd = dev(x, torch.complex128, gpu)
+    n = shape[0] * shape[1]
+    want = O.fft2D(O.Mode.Forward, x)
+    check(afft.fft2D(Mode.Forward, xd), want, torch.complex128, n, "two splits")
+    p = afft.Plan(shape, torch.complex128, "all")
+    assert p.info()["launches"] == 5  # A, rows, transpose; column A, column B
+    y = xd.clone()
+    p.execute(y, out=y)
+    check(y, want, torch.complex128, n, "two splits in place")
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("shape", [(16, 32, 64), (1, 1, 1), (3, 5, 7), (8, 1, 8), (11, 12, 10),
                                    (32, 32, 32), (64, 16, 48)])
