@@ -28,7 +28,7 @@ from . import _lib as L
 from ._lib import AfftError
 
 __all__ = ["Mode", "fft", "fft1D", "fft2D", "fft3D", "fft2D_many", "vjp", "Plan", "AfftError",
-           "RaggedPlan", "spectral_conv", "spectral_conv_vjp", "describe"]
+           "RaggedPlan", "spectral_conv", "spectral_conv_vjp", "describe", "centre"]
 
 
 class Mode(enum.Enum):
@@ -266,3 +266,6 @@ def spectral_conv_vjp(x, H, gy, want_x=True, want_H=True):
             _ptr(gx) if gx is not None else None, _ptr(gH) if gH is not None else None,
             _stream(x.device)), "afft_spectral_conv_vjp")
     return gx, gH
+
+
+from . import centre  # noqa: E402  (DFT/Centre.hs mirror; uses the helpers above)

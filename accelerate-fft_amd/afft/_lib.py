@@ -14,6 +14,7 @@ C64 = 0
 C128 = 1
 FORWARD, REVERSE, INVERSE = 0, 1, 2
 AXES_INNERMOST, AXES_ALL = 0, 1
+REORDER_NONE, CENTRE, SHIFT, ISHIFT = 0, 1, 2, 3
 
 _lib = None
 
@@ -69,6 +70,8 @@ def lib():
     _sig(L.afft_autotune, i, i, i, pi64, i, i)
     _sig(L.afft_tune_save, i, ctypes.c_char_p)
     _sig(L.afft_tune_load, i, ctypes.c_char_p)
+    _sig(L.afft_reorder, i, i, i, i, pi64, vp, vp, vp)
+    _sig(L.afft_fft_centred, i, i, i, i, pi64, i, i, vp, vp, vp)
     _lib = L
     return L
 

@@ -30,11 +30,12 @@ afft_regcol(const FftArgs a)
     const int64_t j = tile * kRegColBlock + threadIdx.x;
     if (j >= a.ntrans) return;
 
-    const V *in = reinterpret_cast<const V *>(a.in) + addr_base(a.ain, j);
+    const int64_t ibase = addr_base(a.ain, j);
+    const V *in = reinterpret_cast<const V *>(a.in) + ibase;
     V reg[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        const V v = c_load(in + (int64_t)i * a.ain.es, (bool)a.nt_in);
+        const V v = sign_apply(a.sg_in, ibase + (int64_t)i * a.ain.es, c_load(in + (int64_t)i * a.ain.es, (bool)a.nt_in));
         reg[i] = a.conj_in ? c_conj(v) : v;
     }
     dft<R>::run(reg);
@@ -57,13 +58,15 @@ afft_regcol(const FftArgs a)
         }
     }
 
-    V *out = reinterpret_cast<V *>(a.out) + addr_base(a.aout, j);
+    const int64_t obase = addr_base(a.aout, j);
+    V *out = reinterpret_cast<V *>(a.out) + obase;
     const Real sc = (Real)a.scale;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         V v = reg[k];
         if (a.conj_out) v = c_conj(v);
-        c_store(out + (int64_t)k * a.aout.es, c_scale(v, sc), (bool)a.nt_out);
+        c_store(out + (int64_t)k * a.aout.es, sign_apply(a.sg_out, obase + (int64_t)k * a.aout.es, c_scale(v, sc)),
+                (bool)a.nt_out);
     }
 }
 
@@ -120,7 +123,8 @@ __global__ void __launch_bounds__(256) afft_transpose(const TransposeArgs a)
     if (o >= a.nbatch) return; // whole workgroup: no barrier is skipped by part of it
     const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
     const V *in = reinterpret_cast<const V *>(a.in) + o * a.rows * a.cols;
-    V *out = reinterpret_cast<V *>(a.out) + (a.toff ? a.toff[o] : o * a.rows * a.cols);
+    const int64_t obase = a.toff ? a.toff[o] : o * a.rows * a.cols;
+    V *out = reinterpret_cast<V *>(a.out) + obase;
 #pragma unroll
     for (int i = 0; i < TILE / RS; ++i) {
         const int64_t r = by * TILE + ty + i * RS, c = bx * TILE + tx;
@@ -134,7 +138,7 @@ __global__ void __launch_bounds__(256) afft_transpose(const TransposeArgs a)
         if (r < a.rows && c < a.cols) {
             V v = tile[tx][ty + i * RS];
             if (a.conj_out) v = c_conj(v);
-            c_store(out + c * a.rows + r, c_scale(v, sc), (bool)a.nt_out);
+            c_store(out + c * a.rows + r, sign_apply(a.sg_out, obase + c * a.rows + r, c_scale(v, sc)), (bool)a.nt_out);
         }
     }
 }

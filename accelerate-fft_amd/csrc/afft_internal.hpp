@@ -44,6 +44,46 @@ __device__ __forceinline__ V power_lookup(const V *hi, const V *lo, int64_t e, i
     return c_mul(a, b);
 }
 
+// Division of 0 <= x < 2^31 by a fixed d >= 1 as q = (x * m) >> sh (Granlund-Montgomery with
+// s = ceil(log2 d), m = floor(2^(31+s) / d) + 1, sh = 31 + s: m*d lies in (2^(31+s), 2^(31+s) + d]
+// and d <= 2^s, which makes the quotient exact for every 31-bit x; x*m < 2^63).
+struct FastDiv {
+    uint64_t m;
+    int sh;
+};
+
+// A sign (-1)^(sum of the multi-index) on the elements of a row-major array of rank <= 3, applied
+// on the load of a plan's first pass (centring the input) or on the store of its last pass: the
+// fused form of DFT/Centre.hs.  div[a] divides by the stride of axis a (the product of the
+// extents inside it); the array has fewer than 2^31 elements.
+struct SignSpec {
+    int on;
+    int rank;
+    int64_t ext[3];
+    FastDiv div[3];
+};
+
+// parity of sum_a coord_a(off): with q_a = floor(off / stride_a), coord_a = q_a - ext_a * q_{a-1},
+// so parity(coord_a) = parity(q_a) ^ (parity(ext_a) & parity(q_{a-1}))
+__device__ __forceinline__ int sign_parity(const SignSpec &s, int64_t off)
+{
+    const uint64_t x = (uint64_t)off;
+    int p = 0;
+    uint64_t qprev = 0;
+    for (int a = 0; a < s.rank; ++a) {
+        const uint64_t q = (x * s.div[a].m) >> s.div[a].sh;
+        p ^= (int)((q ^ ((uint64_t)(s.ext[a] & 1) & qprev)) & 1);
+        qprev = q;
+    }
+    return p;
+}
+
+template <typename V> __device__ __forceinline__ V sign_apply(const SignSpec &s, int64_t off, V v)
+{
+    if (!s.on) return v;
+    return sign_parity(s, off) ? c_make<V>(-v.x, -v.y) : v;
+}
+
 // One batched Stockham launch: ntrans transforms of length pt.n, C of them per workgroup.
 struct FftArgs {
     const void *in;
@@ -68,6 +108,7 @@ struct FftArgs {
     const void *post_lo;
     int xcd_remap;       // map workgroups to tiles XCD-contiguously (strided passes)
     int nt_in, nt_out;   // streaming loads / stores (plan input read once / final output)
+    SignSpec sg_in, sg_out; // centring signs on load (first pass) / on store (last pass)
 };
 
 // Fused single-workgroup Bluestein (chirp-z) launch for a length n that is not 5-smooth:
@@ -88,6 +129,7 @@ struct BluesteinArgs {
     double scale;
     int xcd_remap;
     int nt_in, nt_out;
+    SignSpec sg_in, sg_out;
 };
 
 // Pointwise stages of the multi-kernel Bluestein path (lengths whose padded size M exceeds one
@@ -105,6 +147,7 @@ struct ChirpArgs {
     int conj_in, conj_out;
     double scale;
     int nt_in, nt_out;
+    SignSpec sg;         // pre: sign on the load of x; post: sign on the store of y
 };
 hipError_t launch_chirp(int dtype, const ChirpArgs &a, hipStream_t s);
 
@@ -119,6 +162,7 @@ struct RowArgs {
     double scale;
     int nt_in, nt_out;
     const int64_t *roffs; // element offset of every row (ragged batches), or null: row * N
+    SignSpec sg_in, sg_out;
 };
 
 // afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,
@@ -141,6 +185,7 @@ struct TransposeArgs {
     double scale;
     int nt_out;
     int xcd_remap;
+    SignSpec sg_out;
 };
 hipError_t launch_transpose(int dtype, const TransposeArgs &a, hipStream_t s);
 

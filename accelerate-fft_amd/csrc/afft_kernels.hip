@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         const bool nt = a.nt_in;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            V v = c_load(in + base + (int64_t)(t + i * T) * a.ain.es, nt);
+            const int64_t off = base + (int64_t)(t + i * T) * a.ain.es;
+            V v = sign_apply(a.sg_in, off, c_load(in + off, nt));
             reg[i] = a.conj_in ? c_conj(v) : v;
         }
     } else {
@@ -93,7 +94,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
                                           reinterpret_cast<const V *>(a.post_lo), e, a.post_shift));
             }
             if (a.conj_out) v = c_conj(v);
-            c_store(out + base + (int64_t)k * a.aout.es, c_scale(v, sc), a.nt_out);
+            const int64_t off = base + (int64_t)k * a.aout.es;
+            c_store(out + off, sign_apply(a.sg_out, off, c_scale(v, sc)), a.nt_out);
         }
     }
 }
@@ -129,7 +131,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
             const int m = t + i * T;
             V v = c_make<V>(0, 0);
             if (m < a.n) {
-                v = c_load(in + base + (int64_t)m * a.ain.es, (bool)a.nt_in);
+                const int64_t off = base + (int64_t)m * a.ain.es;
+                v = sign_apply(a.sg_in, off, c_load(in + off, (bool)a.nt_in));
                 if (a.conj_in) v = c_conj(v);
                 v = c_mul(v, chirp[m]);
             }
@@ -157,7 +160,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
             if (m < a.n) {
                 V v = c_mul(c_conj(reg[i]), chirp[m]);
                 if (a.conj_out) v = c_conj(v);
-                c_store(out + base + (int64_t)m * a.aout.es, c_scale(v, sc), (bool)a.nt_out);
+                const int64_t off = base + (int64_t)m * a.aout.es;
+                c_store(out + off, sign_apply(a.sg_out, off, c_scale(v, sc)), (bool)a.nt_out);
             }
         }
     }
@@ -182,7 +186,8 @@ __global__ void __launch_bounds__(256) afft_chirp(const ChirpArgs a)
             V *aux = reinterpret_cast<V *>(a.out);
             V v = c_make<V>(0, 0);
             if (m < a.n) {
-                v = c_load(x + addr_base(a.a, j) + m * a.a.es, (bool)a.nt_in);
+                const int64_t off = addr_base(a.a, j) + m * a.a.es;
+                v = sign_apply(a.sg, off, c_load(x + off, (bool)a.nt_in));
                 if (a.conj_in) v = c_conj(v);
                 v = c_mul(v, chirp[m]);
             }
@@ -196,7 +201,8 @@ __global__ void __launch_bounds__(256) afft_chirp(const ChirpArgs a)
             V *y = reinterpret_cast<V *>(a.out);
             V v = c_mul(c_conj(aux[j * a.M + m]), chirp[m]);
             if (a.conj_out) v = c_conj(v);
-            c_store(y + addr_base(a.a, j) + m * a.a.es, c_scale(v, (Real)a.scale), (bool)a.nt_out);
+            const int64_t off = addr_base(a.a, j) + m * a.a.es;
+            c_store(y + off, sign_apply(a.sg, off, c_scale(v, (Real)a.scale)), (bool)a.nt_out);
         }
     }
 }

@@ -771,8 +771,37 @@ static void *slot_ptr(Slot s, const void *in, void *out, const Plan &plan)
     return nullptr;
 }
 
+static FastDiv make_fastdiv(uint64_t d)
+{
+    int sh = 0;
+    while (((uint64_t)1 << sh) < d) ++sh;
+    FastDiv f;
+    f.m = (((uint64_t)1 << (31 + sh)) / d) + 1;
+    f.sh = 31 + sh;
+    return f;
+}
+
+bool make_sign_spec(int rank, const int64_t *extents, SignSpec &out)
+{
+    out = SignSpec{};
+    if (rank < 1 || rank > 3) return false;
+    int64_t total = 1;
+    for (int a = 0; a < rank; ++a) total *= extents[a];
+    if (total >= ((int64_t)1 << 31)) return false;
+    out.on = 1;
+    out.rank = rank;
+    int64_t stride = total;
+    for (int a = 0; a < rank; ++a) {
+        out.ext[a] = extents[a];
+        stride = extents[a] ? stride / extents[a] : 1;
+        out.div[a] = make_fastdiv((uint64_t)std::max<int64_t>(1, stride));
+    }
+    return true;
+}
+
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
-              hipStream_t s, const int64_t *in_rows, const int64_t *out_rows)
+              hipStream_t s, const int64_t *in_rows, const int64_t *out_rows, const SignSpec *sg_in,
+              const SignSpec *sg_out)
 {
     if ((in_rows || out_rows) && (plan.rank != 2 || plan.axes != AFFT_AXES_INNERMOST))
         return AFFT_ERROR_INTERNAL;
@@ -812,6 +841,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.scale = last ? scale : 1.0;
             if (l.src == Slot::kIn) a.ain.toff = in_rows;
             if (l.dst == Slot::kOut) a.aout.toff = out_rows;
+            if (first && sg_in) a.sg_in = *sg_in;
+            if (last && sg_out) a.sg_out = *sg_out;
             e = l.kind == Launch::kRegCol ? launch_regcol(plan.dtype, a, l.grid, s)
                                           : launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
         } else if (l.kind == Launch::kRows) {
@@ -828,6 +859,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
                 if (in_rows != out_rows || l.src != Slot::kIn || l.dst != Slot::kOut) return AFFT_ERROR_INTERNAL;
                 a.roffs = in_rows;
             }
+            if (first && sg_in) a.sg_in = *sg_in;
+            if (last && sg_out) a.sg_out = *sg_out;
             e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
         } else if (l.kind == Launch::kTranspose) {
             TransposeArgs a = l.ta;
@@ -838,6 +871,7 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.scale = last ? scale : 1.0;
             if (l.src == Slot::kIn || first) return AFFT_ERROR_INTERNAL;
             if (l.dst == Slot::kOut) a.toff = out_rows;
+            if (last && sg_out) a.sg_out = *sg_out;
             e = launch_transpose(plan.dtype, a, s);
         } else if (l.kind == Launch::kChirp) {
             ChirpArgs a = l.ca;
@@ -850,6 +884,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.scale = last ? scale : 1.0;
             if (l.src == Slot::kIn) a.a.toff = in_rows;
             if (l.dst == Slot::kOut) a.a.toff = out_rows;
+            if (first && sg_in) a.sg = *sg_in;
+            if (last && sg_out) a.sg = *sg_out;
             e = launch_chirp(plan.dtype, a, s);
         } else {
             BluesteinArgs a = l.ba;
@@ -862,6 +898,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             a.scale = last ? scale : 1.0;
             if (l.src == Slot::kIn) a.ain.toff = in_rows;
             if (l.dst == Slot::kOut) a.aout.toff = out_rows;
+            if (first && sg_in) a.sg_in = *sg_in;
+            if (last && sg_out) a.sg_out = *sg_out;
             e = launch_bluestein(plan.dtype, a, l.block, l.grid, l.lds, s);
         }
         if (e != hipSuccess) st = AFFT_ERROR_HIP;

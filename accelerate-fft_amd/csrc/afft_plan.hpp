@@ -70,9 +70,14 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
                const DeviceInfo &dev, hipStream_t s);
 void free_plan(Plan &plan);
 // in_rows / out_rows (optional, plans of shape (count, L) along the innermost axis only): element
-// offsets of the count rows in d_in / d_out, for the rows of one length of a ragged batch
+// offsets of the count rows in d_in / d_out, for the rows of one length of a ragged batch.
+// sg_in / sg_out (optional): a centring sign applied on the first pass's loads / the last pass's
+// stores (fused DFT/Centre.hs operations).
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
-              hipStream_t s, const int64_t *in_rows = nullptr, const int64_t *out_rows = nullptr);
+              hipStream_t s, const int64_t *in_rows = nullptr, const int64_t *out_rows = nullptr,
+              const SignSpec *sg_in = nullptr, const SignSpec *sg_out = nullptr);
+// the sign (-1)^(sum of the multi-index) over a row-major array of rank <= 3 (false: too large)
+bool make_sign_spec(int rank, const int64_t *extents, SignSpec &out);
 int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
 int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
 int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream);

@@ -201,6 +201,11 @@ afft_rows(const RowArgs a)
         V reg[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
+        if (a.sg_in.on && g * C + c < nrows) {
+            const int64_t rb = roffs ? roffs[g * C + c] : (g * C + c) * N;
+#pragma unroll
+            for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_in, rb + t + i * T, reg[i]);
+        }
         if constexpr (PF) {
             __builtin_amdgcn_sched_barrier(0);
             load_tile(g + gridDim.x);
@@ -249,6 +254,11 @@ afft_rows(const RowArgs a)
         });
 
         const int64_t row = g * C + c;
+        if (row < nrows && a.sg_out.on) {
+            const int64_t rb = roffs ? roffs[row] : row * N;
+#pragma unroll
+            for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_out, rb + t + i * T, reg[i]);
+        }
         if (row < nrows) {
             V *q = out + (roffs ? roffs[row] : row * N) + t;
             if (nt_out) {

@@ -101,6 +101,25 @@ int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, const void 
 int afft_fft2D_many(int mode, int dtype, int64_t batch, int64_t h, int64_t w, const void *d_in,
                     void *d_out, void *stream);
 
+/* Centring and shifting (DFT/Centre.hs:36-164) of a rank 1-3 array of fewer than 2^31 elements.
+ * `which`: AFFT_CENTRE  x * (-1)^(sum of the multi-index)                 (centre1D/2D/3D, :36-65)
+ *          AFFT_SHIFT   y[i] = x[(i + ceil(n/2)) mod n] along every axis   (shift1D/2D/3D)
+ *          AFFT_ISHIFT  y[i] = x[(i + floor(n/2)) mod n] along every axis  (ishift1D/2D/3D)
+ * Extension: the reference computes these as separate Accelerate arrays. */
+#define AFFT_REORDER_NONE 0
+#define AFFT_CENTRE 1
+#define AFFT_SHIFT 2
+#define AFFT_ISHIFT 3
+int afft_reorder(int which, int dtype, int rank, const int64_t *extents, const void *d_in, void *d_out,
+                 void *stream);
+/* post(fftND(mode, pre(x))) over all axes of a rank 1-3 array (fft1D/2D/3D of FFT.hs:92-173 with
+ * the Centre.hs operations on either side), pre / post as `which` above.  A centring, and a shift
+ * or ishift when every extent is even, is folded into the transform's first load or last store
+ * (no extra pass: shift(F x) = F(x (-1)^m) and F(shift x) = (-1)^k F x for even n); a shift over
+ * an odd extent adds one reordering pass. */
+int afft_fft_centred(int mode, int dtype, int rank, const int64_t *extents, int pre, int post,
+                     const void *d_in, void *d_out, void *stream);
+
 /* ---- explicit plans --------------------------------------------------------------------- */
 
 /* Build an immutable plan: kernels chosen, twiddles generated on the device in FP64 and rounded

@@ -15,6 +15,8 @@ the reference's front-end semantics on numpy arrays:
                (FFT.hs:119-173; LLVM/Native.hs:84-102 axes [0,1] and [0,1,2])
 * ``dft_brute`` -- the O(N^2) definition of src/Data/Array/Accelerate/Math/DFT.hs:68-111 with
                the roots of DFT/Roots.hs:26-51, in long double.
+* ``centre`` / ``shift`` / ``ishift`` -- src/Data/Array/Accelerate/Math/DFT/Centre.hs:36-164, as
+               an index-by-index restatement of the generate / backpermute definitions.
 
 Arrays are numpy ``complex128`` (FP64 throughout) in Accelerate's row-major order, innermost
 index fastest; ``complex64`` inputs are promoted, so the oracle consumes the rounded values.
@@ -197,3 +199,37 @@ def rel_l2(got, want):
     if den == 0.0:
         return float(num)
     return float(num / den)
+
+
+# ----------------------------------------------------------------------------------------------
+# DFT/Centre.hs, restated element by element from the index functions (not with numpy's
+# fftshift, which the tests use as an independent check).
+
+def centre(arr):
+    """centre1D/2D/3D (Centre.hs:36-65): x[ix] * ((-1) ** sum(ix) :+ 0)."""
+    x = np.asarray(arr, dtype=np.complex128)
+    out = np.empty_like(x)
+    for ix in np.ndindex(*x.shape):
+        out[ix] = x[ix] * (-1.0) ** sum(ix)
+    return out
+
+
+def _roll_back(arr, shift_of):
+    """backpermute sh p arr with p(ix)_a = (ix_a + shift_of(n_a)) `rem` n_a."""
+    x = np.asarray(arr, dtype=np.complex128)
+    out = np.empty_like(x)
+    for ix in np.ndindex(*x.shape):
+        src = tuple((i + shift_of(n)) % n for i, n in zip(ix, x.shape))
+        out[ix] = x[src]
+    return out
+
+
+def shift(arr):
+    """shift1D/2D/3D (Centre.hs:70-82, 104-123): shift_a = n_a `quot` 2 + boolToInt (odd n_a)."""
+    return _roll_back(arr, lambda n: n // 2 + (n % 2))
+
+
+def ishift(arr):
+    """ishift1D/2D/3D (Centre.hs:84-101, 125-164): shift_a = n_a `quot` 2."""
+    return _roll_back(arr, lambda n: n // 2)
+

@@ -245,3 +245,42 @@ def test_float_variant_accuracy(n):
     want = O.fft_rows(xf.astype(np.complex128), -1)
     got = O.fft_rows_f32(xf, -1)
     assert O.rel_l2(got, want) <= 1e-6 * max(1.0, math.log2(max(n, 2)))
+
+
+# ----------------------------------------------------------------------------------------------
+# DFT/Centre.hs restatement
+
+@pytest.mark.parametrize("shape", [(1,), (5,), (8,), (3, 4), (4, 5), (2, 3, 5), (4, 4, 2)])
+def test_centre_shift_against_numpy_and_identities(shape):
+    x = O.uniform(31, shape)
+    # shift is numpy's fftshift and ishift its inverse (an independent library routine)
+    np.testing.assert_array_equal(O.shift(x), np.fft.fftshift(x))
+    np.testing.assert_array_equal(O.ishift(x), np.fft.ifftshift(x))
+    # ishift . shift = id (Centre.hs:84-86)
+    np.testing.assert_array_equal(O.ishift(O.shift(x)), x)
+    # centre is an involution with the sign of the index parity
+    np.testing.assert_array_equal(O.centre(O.centre(x)), x)
+    signs = np.indices(shape).sum(axis=0) % 2
+    np.testing.assert_array_equal(O.centre(x), np.where(signs == 1, -x, x))
+    # fft(centre(X)) = shift(fft(X)) (Centre.hs:17-19) when every extent is even
+    F = O.fft_axes(x, tuple(range(len(shape))), -1)
+    if all(n % 2 == 0 for n in shape):
+        np.testing.assert_allclose(O.fft_axes(O.centre(x), tuple(range(len(shape))), -1), O.shift(F),
+                                   rtol=0, atol=1e-12 * np.abs(F).max())
+
+
+def test_shift_known_answers():
+    v = np.arange(5, dtype=np.complex128)
+    np.testing.assert_array_equal(O.shift(v).real, [3, 4, 0, 1, 2])   # roll by ceil(5/2) = 3
+    np.testing.assert_array_equal(O.ishift(v).real, [2, 3, 4, 0, 1])  # roll by floor(5/2) = 2
+    np.testing.assert_array_equal(O.centre(v).real, [0, -1, 2, -3, 4])
+
+
+def test_planted_centre_mistakes_fail():
+    x = O.uniform(5, (5, 6))
+    # y[i] = x[(i + s) mod n] is np.roll by -s; the right shift rolls the odd axis by ceil(5/2) = 3
+    np.testing.assert_array_equal(O.shift(x), np.roll(x, (-3, -3), axis=(0, 1)))
+    wrong_shift = np.roll(x, (-2, -3), axis=(0, 1))    # floor instead of ceil on the odd axis
+    assert not np.array_equal(O.shift(x), wrong_shift)
+    wrong_centre = x * (-1.0) ** np.indices(x.shape)[1]  # only the innermost index
+    assert not np.array_equal(O.centre(x), wrong_centre)
