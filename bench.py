@@ -20,6 +20,9 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
              time and roofline fraction; the ragged batch and the spectral convolution also
              against the plain per-call path on the same GPU.
 
+  Before the W warmup steps, untimed steps run for 50 ms so that a fresh process's clocks have
+  ramped up (the first ~30 calls otherwise run 5-8% slower).
+
   --profile-headline runs only the headline plan (and a plain device copy of the same array,
              the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
   --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral) for
@@ -295,7 +298,13 @@ def main():
         print(json.dumps({"profile_headline": True, "steps": args.steps}), flush=True)
         return
 
-    # headline: K timed steps between barriers (synchronize) on both sides
+    # settle: untimed steps until 50 ms have passed, so that the clocks of a freshly started
+    # process have ramped up before the W warmup steps (the first ~30 calls otherwise run 5-8%
+    # slower); then K timed steps between barriers (synchronize) on both sides
+    settle_end = time.perf_counter() + 0.05
+    while time.perf_counter() < settle_end:
+        step()
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
