@@ -28,7 +28,7 @@ from . import _lib as L
 from ._lib import AfftError
 
 __all__ = ["Mode", "fft", "fft1D", "fft2D", "fft3D", "fft2D_many", "vjp", "Plan", "AfftError",
-           "RaggedPlan", "spectral_conv", "spectral_conv_vjp", "describe", "centre"]
+           "RaggedPlan", "spectral_conv", "spectral_conv_vjp", "describe", "centre", "dft"]
 
 
 class Mode(enum.Enum):
@@ -269,3 +269,4 @@ def spectral_conv_vjp(x, H, gy, want_x=True, want_H=True):
 
 
 from . import centre  # noqa: E402  (DFT/Centre.hs mirror; uses the helpers above)
+from . import dft  # noqa: E402  (Math/DFT.hs mirror)

@@ -72,6 +72,9 @@ def lib():
     _sig(L.afft_tune_load, i, ctypes.c_char_p)
     _sig(L.afft_reorder, i, i, i, i, pi64, vp, vp, vp)
     _sig(L.afft_fft_centred, i, i, i, i, pi64, i, i, vp, vp, vp)
+    _sig(L.afft_dft, i, i, i, i, pi64, vp, vp, vp)
+    _sig(L.afft_dftG, i, i, i, pi64, vp, vp, vp, vp)
+    _sig(L.afft_dftGS, i, i, i, pi64, pi64, vp, vp, vp, vp)
     _lib = L
     return L
 

@@ -120,6 +120,23 @@ int afft_reorder(int which, int dtype, int rank, const int64_t *extents, const v
 int afft_fft_centred(int mode, int dtype, int rank, const int64_t *extents, int pre, int post,
                      const void *d_in, void *d_out, void *stream);
 
+/* Data.Array.Accelerate.Math.DFT (DFT.hs:42-111), O(N^2) on the device, for validation and for
+ * extents the FFT would pad.  As the reference's dftG, the sum runs over EVERY element of the array
+ * (DFT.hs:68-92): out[ix] = sum_i in[i] * roots[sh_i :. (k * n_i) mod l], k the innermost index of
+ * ix; for a vector that is the DFT, for a higher rank each row receives the same sums.
+ *   afft_dft    dft (Forward, rootsOfUnity), idft (Inverse: inverse roots, divided by n), or the
+ *               unnormalised inverse (Reverse); roots from Roots.hs:26-51 with the phase reduced in
+ *               integers and evaluated in FP64
+ *   afft_dftG   caller-supplied roots of the input's shape
+ *   afft_dftGS  the single value at `index` (rank entries) into d_out[0]
+ * The innermost extent must be below 2^31; out may not alias the input. */
+int afft_dft(int mode, int dtype, int rank, const int64_t *extents, const void *d_in, void *d_out,
+             void *stream);
+int afft_dftG(int dtype, int rank, const int64_t *extents, const void *d_roots, const void *d_in,
+              void *d_out, void *stream);
+int afft_dftGS(int dtype, int rank, const int64_t *extents, const int64_t *index, const void *d_roots,
+               const void *d_in, void *d_out, void *stream);
+
 /* ---- explicit plans --------------------------------------------------------------------- */
 
 /* Build an immutable plan: kernels chosen, twiddles generated on the device in FP64 and rounded

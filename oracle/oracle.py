@@ -17,6 +17,8 @@ the reference's front-end semantics on numpy arrays:
                the roots of DFT/Roots.hs:26-51, in long double.
 * ``centre`` / ``shift`` / ``ishift`` -- src/Data/Array/Accelerate/Math/DFT/Centre.hs:36-164, as
                an index-by-index restatement of the generate / backpermute definitions.
+* ``dftG`` -- src/Data/Array/Accelerate/Math/DFT.hs:63-92, the extended-array sum over every
+               element, in FP64 with the index product reduced in integers.
 
 Arrays are numpy ``complex128`` (FP64 throughout) in Accelerate's row-major order, innermost
 index fastest; ``complex64`` inputs are promoted, so the oracle consumes the rounded values.
@@ -232,4 +234,23 @@ def shift(arr):
 def ishift(arr):
     """ishift1D/2D/3D (Centre.hs:84-101, 125-164): shift_a = n_a `quot` 2."""
     return _roll_back(arr, lambda n: n // 2)
+
+
+# ----------------------------------------------------------------------------------------------
+# Math/DFT.hs
+
+def dftG(roots, arr):
+    """DFT.hs:63-92: out[ix] = sum over every linear index i of arr[i] * roots[sh_i :. (k*n_i) mod l],
+    (sh_i :. n_i) = fromIndex (shape arr) i, k = indexHead ix."""
+    x = np.asarray(arr, dtype=np.complex128)
+    r = np.asarray(roots, dtype=np.complex128)
+    l = x.shape[-1]
+    xf = x.reshape(-1, l)
+    rf = r.reshape(-1, l)
+    n = np.arange(l, dtype=np.int64)
+    vals = np.empty(l, dtype=np.complex128)
+    for k in range(l):
+        e = (k * n) % l
+        vals[k] = np.sum(xf * rf[:, e])
+    return np.broadcast_to(vals, x.shape).copy()
 

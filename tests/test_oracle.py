@@ -284,3 +284,35 @@ def test_planted_centre_mistakes_fail():
     assert not np.array_equal(O.shift(x), wrong_shift)
     wrong_centre = x * (-1.0) ** np.indices(x.shape)[1]  # only the innermost index
     assert not np.array_equal(O.centre(x), wrong_centre)
+
+
+# ----------------------------------------------------------------------------------------------
+# Math/DFT.hs restatement
+
+def _std_roots(shape, sign):
+    n = shape[-1]
+    k = 2 * np.pi * np.arange(n) / n
+    return np.broadcast_to(np.cos(k) + 1j * sign * np.sin(k), shape)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 31, 64])
+def test_dftG_vector_is_the_dft(n):
+    x = O.uniform(n, (n,))
+    for sign in (-1, 1):
+        got = O.dftG(_std_roots((n,), sign), x)
+        np.testing.assert_allclose(got, O.dft_brute(x, sign), rtol=0, atol=1e-12 * max(1, n))
+
+
+def test_dftG_higher_rank_sums_every_row():
+    # the reference folds over shapeSize elements (DFT.hs:79-92): every row gets the DFT of the
+    # row sum when all rows share the roots
+    x = O.uniform(3, (4, 12))
+    got = O.dftG(_std_roots(x.shape, -1), x)
+    want = O.dft_brute(x.sum(axis=0), -1)
+    for r in range(4):
+        np.testing.assert_allclose(got[r], want, rtol=0, atol=1e-12)
+    # and per-row roots weight each row by its own roots
+    roots = _std_roots(x.shape, -1) * np.arange(1, 5)[:, None]
+    got2 = O.dftG(roots, x)
+    want2 = sum(O.dft_brute(x[r] * (r + 1), -1) for r in range(4))
+    np.testing.assert_allclose(got2[0], want2, rtol=0, atol=1e-11)
