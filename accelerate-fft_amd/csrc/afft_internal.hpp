@@ -163,10 +163,15 @@ struct RowArgs {
     int nt_in, nt_out;
     const int64_t *roffs; // element offset of every row (ragged batches), or null: row * N
     SignSpec sg_in, sg_out;
+    // transposed store (variant kRowTransposed): row r = (o, k1) of n1 rows per transform, its
+    // element k2 stored at (tr_toff ? tr_toff[o] : o * N * n1) + k1 + n1 * k2
+    int64_t tr_n1;
+    const int64_t *tr_toff;
 };
 
 // afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,
 // LDS bytes, rows per tile, resident workgroups per CU, and the launcher.
+constexpr int kRowTransposed = -2; // the variant index of a length's transposed-store form
 int row_variants(int dtype, int n);
 int row_block(int dtype, int n, int variant);
 size_t row_lds_bytes(int dtype, int n, int variant);

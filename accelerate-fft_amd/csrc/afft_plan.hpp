@@ -41,8 +41,13 @@ struct Launch {
     size_t lds;
 };
 
+// Plan::flags: the plan runs on gathered rows that start at arbitrary offsets (ragged batches),
+// so short row segments of a transposed store would straddle cache lines
+constexpr int kPlanUnalignedRows = 1;
+
 struct Plan {
     int dtype = 0, rank = 0, axes = 0, device = 0;
+    int flags = 0;
     std::vector<int64_t> extents;
     int64_t total = 0;
     std::vector<Launch> launches;

@@ -173,6 +173,8 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             lp->L = kv.first;
             lp->small = (int64_t)cnt * kv.first < kSmallLongElems;
             const int64_t ext[2] = {cnt, kv.first};
+            // rows packed back to back start at arbitrary 8- or 16-byte offsets
+            lp->plan.flags = kPlanUnalignedRows;
             const int st = build_plan(lp->plan, rp.dtype, 2, ext, AFFT_AXES_INNERMOST, rp.device, dev, s);
             rp.longs.push_back(std::move(lp));
             if (st) return st;
@@ -235,7 +237,8 @@ static int run_long(const RaggedLong &lp, int mode, const void *in, void *out, h
 static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(rp.launch_mu);
-    if (hipEventRecord(rp.fork, s) != hipSuccess || hipStreamWaitEvent(rp.side, rp.fork, 0) != hipSuccess)
+    hipStream_t side = rp.side;
+    if (hipEventRecord(rp.fork, s) != hipSuccess || hipStreamWaitEvent(side, rp.fork, 0) != hipSuccess)
         return AFFT_ERROR_HIP;
     int st = AFFT_SUCCESS;
     if (rp.nitems > 0) {
@@ -248,14 +251,14 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
         a.moffs = rp.d_offsets;
         a.conj = mode == AFFT_FORWARD ? 0 : 1;
         a.inverse = mode == AFFT_INVERSE ? 1 : 0;
-        if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, rp.side) != hipSuccess) st = AFFT_ERROR_HIP;
+        if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, side) != hipSuccess) st = AFFT_ERROR_HIP;
     }
     for (const auto &lp : rp.longs)
-        if (!st && lp->small) st = run_long(*lp, mode, in, out, rp.side);
+        if (!st && lp->small) st = run_long(*lp, mode, in, out, side);
     for (const auto &lp : rp.longs)
         if (!st && !lp->small) st = run_long(*lp, mode, in, out, s);
     // join even after an error, so that the side stream never runs ahead of the caller's
-    if (hipEventRecord(rp.join, rp.side) != hipSuccess || hipStreamWaitEvent(s, rp.join, 0) != hipSuccess)
+    if (hipEventRecord(rp.join, side) != hipSuccess || hipStreamWaitEvent(s, rp.join, 0) != hipSuccess)
         st = st ? st : AFFT_ERROR_HIP;
     return st;
 }

@@ -27,6 +27,8 @@
 
 #include <utility>
 
+#include <algorithm>
+
 #include "afft_device.hpp"
 #include "afft_internal.hpp"
 
@@ -125,7 +127,7 @@ template <int N, int E> struct RowShape {
     static constexpr int BLOCK = C * T;
 };
 
-template <typename Real, int N, int E, bool PF, int W, bool LD, int... Rs>
+template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, int... Rs>
 __global__ void __launch_bounds__((RowShape<N, E>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W, 8)))
 afft_rows(const RowArgs a)
 {
@@ -253,20 +255,49 @@ afft_rows(const RowArgs a)
             }
         });
 
-        const int64_t row = g * C + c;
-        if (row < nrows && a.sg_out.on) {
-            const int64_t rb = roffs ? roffs[row] : row * N;
+        if constexpr (TR) {
+            // transposed store (the last pass of a six-step transform): the C rows of the tile
+            // are consecutive k1 of one transform, and X[k1 + n1 k2] goes to k1 + n1 k2.  The
+            // results are regrouped through LDS (element k2 of row c at k2 (C+1) + c) and re-read
+            // with the row varying fastest across lanes, so each store writes C neighbouring k1.
+            constexpr int C1 = C + 1;
+            lds_sync(); // every row of the tile is past its last exchange read
 #pragma unroll
-            for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_out, rb + t + i * T, reg[i]);
-        }
-        if (row < nrows) {
-            V *q = out + (roffs ? roffs[row] : row * N) + t;
-            if (nt_out) {
+            for (int i = 0; i < E; ++i) lds[(tl + i * T) * C1 + c] = reg[i];
+            lds_sync();
+            const int c2 = threadIdx.x % C, t2 = threadIdx.x / C;
+            const int64_t row2 = g * C + c2;
+            if (row2 < nrows) {
+                const int64_t n1 = a.tr_n1;
+                const int64_t o = row2 / n1, k1 = row2 - o * n1;
+                const int64_t ob = (a.tr_toff ? a.tr_toff[o] : o * (int64_t)N * n1) + k1;
 #pragma unroll
-                for (int i = 0; i < E; ++i) c_store_nt(q + i * T, c_scale(cout ? c_conj(reg[i]) : reg[i], sc));
-            } else {
+                for (int i = 0; i < E; ++i) {
+                    const int64_t k2 = t2 + i * T;
+                    V v = lds[k2 * C1 + c2];
+                    v = c_scale(cout ? c_conj(v) : v, sc);
+                    v = sign_apply(a.sg_out, ob + n1 * k2, v);
+                    if (nt_out) c_store_nt(out + ob + n1 * k2, v);
+                    else out[ob + n1 * k2] = v;
+                }
+            }
+            lds_sync(); // the regroup buffer is read before the next tile's first exchange
+        } else {
+            const int64_t row = g * C + c;
+            if (row < nrows && a.sg_out.on) {
+                const int64_t rb = roffs ? roffs[row] : row * N;
 #pragma unroll
-                for (int i = 0; i < E; ++i) q[i * T] = c_scale(cout ? c_conj(reg[i]) : reg[i], sc);
+                for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_out, rb + t + i * T, reg[i]);
+            }
+            if (row < nrows) {
+                V *q = out + (roffs ? roffs[row] : row * N) + t;
+                if (nt_out) {
+#pragma unroll
+                    for (int i = 0; i < E; ++i) c_store_nt(q + i * T, c_scale(cout ? c_conj(reg[i]) : reg[i], sc));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < E; ++i) q[i * T] = c_scale(cout ? c_conj(reg[i]) : reg[i], sc);
+                }
             }
         }
     }
@@ -286,10 +317,14 @@ struct RowEntry {
     bool pf;
     RowFn fn;
     int block;
+    bool tr;
 };
 
 #define AFFT_ROW(REAL, DT, N, E, PF, W, LD, ...)                                                 \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, __VA_ARGS__>, RowShape<N, E>::BLOCK}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, false}
+// the transposed-store form, one per length (selected with variant kRowTransposed)
+#define AFFT_ROW_TR(REAL, DT, N, E, PF, W, LD, ...)                                              \
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, __VA_ARGS__>, RowShape<N, E>::BLOCK, true}
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
@@ -317,6 +352,12 @@ const RowEntry kRows[] = {
     AFFT_ROW(float, 0, 640, 20, true, 2, true, 4, 4, 2, 4, 5),
     AFFT_ROW(float, 0, 768, 24, true, 2, true, 4, 8, 8, 3),
     AFFT_ROW(float, 0, 1536, 24, true, 2, true, 8, 8, 8, 3),
+    AFFT_ROW_TR(float, 0, 160, 20, true, 2, true, 2, 4, 4, 5),
+    AFFT_ROW_TR(float, 0, 320, 20, true, 2, true, 4, 4, 4, 5),
+    AFFT_ROW_TR(float, 0, 384, 24, true, 2, true, 4, 4, 8, 3),
+    AFFT_ROW_TR(float, 0, 512, 16, true, 2, true, 16, 16, 2),
+    AFFT_ROW_TR(float, 0, 640, 20, true, 2, true, 4, 4, 2, 4, 5),
+    AFFT_ROW_TR(float, 0, 768, 24, true, 2, true, 4, 8, 8, 3),
     AFFT_ROW(double, 1, 256, 8, true, 2, false, 8, 8, 4),
     AFFT_ROW(double, 1, 512, 8, true, 2, false, 8, 8, 8),
     AFFT_ROW(double, 1, 1024, 8, true, 2, false, 8, 8, 8, 2),
@@ -329,10 +370,15 @@ const RowEntry kRows[] = {
 
 const RowEntry *find_row(int dtype, int n, int variant)
 {
+    if (variant == kRowTransposed) {
+        for (const RowEntry &e : kRows)
+            if (e.dtype == dtype && e.n == n && e.tr) return &e;
+        return nullptr;
+    }
     const RowEntry *first = nullptr;
     int seen = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n) {
+        if (e.dtype == dtype && e.n == n && !e.tr) {
             if (!first) first = &e;
             if (seen++ == variant) return &e;
         }
@@ -345,7 +391,7 @@ int row_variants(int dtype, int n)
 {
     int v = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n) ++v;
+        if (e.dtype == dtype && e.n == n && !e.tr) ++v;
     return v;
 }
 
@@ -361,7 +407,9 @@ size_t row_lds_bytes(int dtype, int n, int variant)
     if (!e) return 0;
     const int T = e->n / e->E;
     const int C = T >= 256 ? 1 : 256 / T;
-    return (size_t)C * lds_padded_len(e->n) * (dtype == 0 ? 8 : 16);
+    size_t elems = (size_t)C * lds_padded_len(e->n);
+    if (e->tr) elems = std::max(elems, (size_t)e->n * (size_t)(C + 1)); // the regroup buffer
+    return elems * (dtype == 0 ? 8 : 16);
 }
 
 int row_rows_per_tile(int dtype, int n, int variant)
