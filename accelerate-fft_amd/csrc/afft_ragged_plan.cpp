@@ -34,8 +34,11 @@ struct RaggedLong {
     bool small = false;            // too few elements to fill the device: runs on the side stream
 };
 
-// a long class below this many elements leaves most CUs idle and is latency-bound
-constexpr int64_t kSmallLongElems = 1 << 20;
+// A long class below this many elements (64 MiB of FP32 data) is latency- or tail-bound: its one
+// or two launches hold a few rows per CU.  Those classes run on the side stream beside the large
+// ones (measured on the 10,000-transform batch: 1M -> 8M moved the 5120...12288 classes there and
+// took it from 550 to 465 us).
+constexpr int64_t kSmallLongElems = 1 << 23;
 
 struct RaggedPlan {
     int dtype = 0, device = 0;
