@@ -35,8 +35,12 @@ afft_regcol(const FftArgs a)
     V reg[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        const V v = sign_apply(a.sg_in, ibase + (int64_t)i * a.ain.es, c_load(in + (int64_t)i * a.ain.es, (bool)a.nt_in));
+        const V v = c_load(in + (int64_t)i * a.ain.es, (bool)a.nt_in);
         reg[i] = a.conj_in ? c_conj(v) : v;
+    }
+    if (a.sg_in.on) { // apart from the loads, which stay one batch of independent instructions
+#pragma unroll
+        for (int i = 0; i < R; ++i) reg[i] = sign_apply(a.sg_in, ibase + (int64_t)i * a.ain.es, reg[i]);
     }
     dft<R>::run(reg);
 
@@ -61,12 +65,15 @@ afft_regcol(const FftArgs a)
     const int64_t obase = addr_base(a.aout, j);
     V *out = reinterpret_cast<V *>(a.out) + obase;
     const Real sc = (Real)a.scale;
+    if (a.sg_out.on) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) reg[k] = sign_apply(a.sg_out, obase + (int64_t)k * a.aout.es, reg[k]);
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         V v = reg[k];
         if (a.conj_out) v = c_conj(v);
-        c_store(out + (int64_t)k * a.aout.es, sign_apply(a.sg_out, obase + (int64_t)k * a.aout.es, c_scale(v, sc)),
-                (bool)a.nt_out);
+        c_store(out + (int64_t)k * a.aout.es, c_scale(v, sc), (bool)a.nt_out);
     }
 }
 

@@ -65,9 +65,12 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         const bool nt = a.nt_in;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            const int64_t off = base + (int64_t)(t + i * T) * a.ain.es;
-            V v = sign_apply(a.sg_in, off, c_load(in + off, nt));
+            V v = c_load(in + base + (int64_t)(t + i * T) * a.ain.es, nt);
             reg[i] = a.conj_in ? c_conj(v) : v;
+        }
+        if (a.sg_in.on) { // apart from the loads, which stay one batch of independent instructions
+#pragma unroll
+            for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_in, base + (int64_t)(t + i * T) * a.ain.es, reg[i]);
         }
     } else {
 #pragma unroll
@@ -83,6 +86,10 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         const int64_t base = addr_base(a.aout, j);
         const Real sc = (Real)a.scale;
         const int64_t jl = a.post_L ? addr_level(a.ain, j, a.post_level) : 0;
+        if (a.sg_out.on) { // apart from the stores, which stay one batch
+#pragma unroll
+            for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_out, base + (int64_t)(t + i * T) * a.aout.es, reg[i]);
+        }
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int k = t + i * T;
@@ -94,8 +101,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
                                           reinterpret_cast<const V *>(a.post_lo), e, a.post_shift));
             }
             if (a.conj_out) v = c_conj(v);
-            const int64_t off = base + (int64_t)k * a.aout.es;
-            c_store(out + off, sign_apply(a.sg_out, off, c_scale(v, sc)), a.nt_out);
+            c_store(out + base + (int64_t)k * a.aout.es, c_scale(v, sc), a.nt_out);
         }
     }
 }
