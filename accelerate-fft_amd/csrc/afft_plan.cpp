@@ -756,9 +756,6 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
         plan.aux = b.alloc(plan.aux_elems * (int64_t)b.esz());
     }
     if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
-    // every pass that reads the input either runs in place or writes elsewhere before any
-    // pass writes the output, so in-place execution (input == output) needs no private copy
-    plan.inplace_needs_copy = false;
     return b.status;
 }
 
@@ -818,17 +815,10 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
     if ((in_rows || out_rows) && (plan.rank != 2 || plan.axes != AFFT_AXES_INNERMOST))
         return AFFT_ERROR_INTERNAL;
     if (plan.total == 0) return AFFT_SUCCESS;
+    // in place (d_in == d_out) needs no private copy: every pass that reads the input either runs
+    // in place, each workgroup reading all its elements before writing them, or writes the work
+    // buffer, and the permuting passes read only the plan's own buffers
     const void *src_in = d_in;
-    void *tmp = nullptr;
-    if (d_in == d_out && plan.inplace_needs_copy) {
-        const size_t bytes = (size_t)plan.total * (plan.dtype == AFFT_C64 ? 8 : 16);
-        if (hipMallocAsync(&tmp, bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
-        if (hipMemcpyAsync(tmp, d_in, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-            (void)hipFreeAsync(tmp, s);
-            return AFFT_ERROR_HIP;
-        }
-        src_in = tmp;
-    }
     const int conj = direction > 0 ? 1 : 0;
     const size_t n = plan.launches.size();
     // streaming accesses for the plan's input (read once) and its final output (written once);
@@ -920,7 +910,6 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
         }
         if (e != hipSuccess) st = AFFT_ERROR_HIP;
     }
-    if (tmp) (void)hipFreeAsync(tmp, s);
     return st;
 }
 

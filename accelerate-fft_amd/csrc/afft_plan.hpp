@@ -62,7 +62,6 @@ struct Plan {
     void *aux = nullptr;
     int64_t aux2_elems = 0;  // partner of aux for four-step splits inside a Bluestein transform
     void *aux2 = nullptr;
-    bool inplace_needs_copy = false;
 };
 
 // Single-workgroup layout of length n: fewest passes, then E closest to 16; optional limits on
