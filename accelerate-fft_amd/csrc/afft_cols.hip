@@ -111,8 +111,9 @@ hipError_t launch_regcol(int dtype, const FftArgs &a, int64_t grid, hipStream_t 
 
 
 // ----------------------------------------------------------------------------------------------
-// Batched transpose, the last pass of a six-step transform: matrix o (rows x cols, row-major)
-// is written transposed (cols x rows) at out + (toff ? toff[o] : o * rows * cols).  A square
+// Batched strided transpose, the last pass of a six-step transform: matrix o = o1 + nb1 * o2
+// (rows x cols, element (r, c) at in + o1 in_bs1 + o2 in_bs2 + r in_rs + c) is written transposed,
+// element (c, r) at out + o1 out_bs1 + (toff ? toff[o2] : o2 out_bs2) + c out_rs + r.  A square
 // tile goes through LDS so that both the loads (along a row of the input) and the stores (along
 // a row of the output) are contiguous runs of TILE elements; the LDS row pitch TILE + 1 keeps
 // the column-wise LDS reads off a single bank.
@@ -128,24 +129,28 @@ __global__ void __launch_bounds__(256) afft_transpose(const TransposeArgs a)
     const int64_t bx = b % ntx, q = b / ntx;
     const int64_t by = q % nty, o = q / nty;
     if (o >= a.nbatch) return; // whole workgroup: no barrier is skipped by part of it
+    const int64_t o1 = o % a.nb1, o2 = o / a.nb1;
     const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
-    const V *in = reinterpret_cast<const V *>(a.in) + o * a.rows * a.cols;
-    const int64_t obase = a.toff ? a.toff[o] : o * a.rows * a.cols;
+    const V *in = reinterpret_cast<const V *>(a.in) + o1 * a.in_bs1 + o2 * a.in_bs2;
+    const int64_t obase = o1 * a.out_bs1 + (a.toff ? a.toff[o2] : o2 * a.out_bs2);
     V *out = reinterpret_cast<V *>(a.out) + obase;
 #pragma unroll
     for (int i = 0; i < TILE / RS; ++i) {
         const int64_t r = by * TILE + ty + i * RS, c = bx * TILE + tx;
-        if (r < a.rows && c < a.cols) tile[ty + i * RS][tx] = in[r * a.cols + c];
+        if (r < a.rows && c < a.cols) tile[ty + i * RS][tx] = in[r * a.in_rs + c];
     }
     __syncthreads();
     const Real sc = (Real)a.scale;
+    const bool sign = a.sg_out.on;
 #pragma unroll
     for (int i = 0; i < TILE / RS; ++i) {
         const int64_t c = bx * TILE + ty + i * RS, r = by * TILE + tx;
         if (r < a.rows && c < a.cols) {
             V v = tile[tx][ty + i * RS];
             if (a.conj_out) v = c_conj(v);
-            c_store(out + c * a.rows + r, sign_apply(a.sg_out, obase + c * a.rows + r, c_scale(v, sc)), (bool)a.nt_out);
+            v = c_scale(v, sc);
+            if (sign) v = sign_apply(a.sg_out, obase + c * a.out_rs + r, v);
+            c_store(out + c * a.out_rs + r, v, (bool)a.nt_out);
         }
     }
 }

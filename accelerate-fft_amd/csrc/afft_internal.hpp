@@ -179,12 +179,14 @@ int row_rows_per_tile(int dtype, int n, int variant);
 int row_occupancy(int dtype, int n, int variant);
 hipError_t launch_rows(int dtype, int n, int variant, const RowArgs &a, int64_t grid, hipStream_t s);
 
-// Batched transpose (afft_cols.hip): matrix o of nbatch (rows x cols, row-major, contiguous in
-// `in`) stored transposed at out + (toff ? toff[o] : o * rows * cols), conjugated and scaled.
+// Batched strided transpose (afft_cols.hip): matrix o = o1 + nb1 * o2 of nbatch, element (r, c)
+// at in + o1 in_bs1 + o2 in_bs2 + r in_rs + c, stored transposed with (c, r) at
+// out + o1 out_bs1 + (toff ? toff[o2] : o2 out_bs2) + c out_rs + r, conjugated and scaled.
 struct TransposeArgs {
     const void *in;
     void *out;
     int64_t rows, cols, nbatch;
+    int64_t nb1, in_bs1, in_bs2, in_rs, out_bs1, out_bs2, out_rs;
     const int64_t *toff;
     int conj_out;
     double scale;

@@ -492,15 +492,27 @@ struct Builder {
         return variant;
     }
 
+    // every matrix (rows x cols, contiguous, back to back) transposed
     void add_transpose(int64_t nbatch, int64_t rows, int64_t cols, Slot src, Slot dst)
+    {
+        TransposeArgs t{};
+        t.rows = rows;
+        t.cols = cols;
+        t.nbatch = nbatch;
+        t.nb1 = 1;
+        t.in_bs2 = t.out_bs2 = rows * cols;
+        t.in_rs = cols;
+        t.out_rs = rows;
+        add_transpose(t, src, dst);
+    }
+
+    void add_transpose(const TransposeArgs &t, Slot src, Slot dst)
     {
         Launch l{};
         l.kind = Launch::kTranspose;
         l.src = src;
         l.dst = dst;
-        l.ta.rows = rows;
-        l.ta.cols = cols;
-        l.ta.nbatch = nbatch;
+        l.ta = t;
         l.ta.scale = 1.0;
         l.ta.xcd_remap = !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
         plan.launches.push_back(l);

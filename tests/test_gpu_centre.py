@@ -76,6 +76,7 @@ def test_reference_identity_on_the_gpu(gpu):
     (torch.complex128, (40960,)),       # balanced four-step
     (torch.complex128, (64, 48, 80)),   # rows then strided LDS passes
     (torch.complex64, (2048, 64)),      # split strided columns
+    (torch.complex64, (1 << 20,)),      # balanced four-step of 1024 x 1024
 ])
 def test_fused_signs_on_every_pass_kind(gpu, dtype, shape):
     """The fused forms against the unfused composition of standalone reorders and plain

@@ -154,7 +154,9 @@ def test_fft3D(gpu, dtype, shape):
 
 @pytest.mark.parametrize("dtype,n", [(torch.complex64, 32768), (torch.complex64, 1 << 18),
                                      (torch.complex64, 3 * 8192 * 4), (torch.complex128, 16384),
-                                     (torch.complex128, 40960), (torch.complex64, 1 << 22)])
+                                     (torch.complex128, 40960), (torch.complex64, 1 << 22),
+                                     (torch.complex64, 1 << 20), (torch.complex128, 1 << 20),
+                                     (torch.complex64, 5 * 64 * 64 * 512)])
 def test_long_rows_four_step(gpu, dtype, n):
     x = O.uniform(n, (2, n))
     xd = dev(x, dtype, gpu)
