@@ -615,6 +615,91 @@ struct Builder {
         add_transpose(o, n1, n2, cur, next);
     }
 
+    // Factors of o contiguous rows n = n1 * m1 * m2 for a six-step whose rows n2 = m1 * m2 have
+    // no compiled row kernel: a register column length n1, a strided single-pass length m1 whose
+    // tiles read whole lines, and a row length m2 with a transposed-store form.  The register
+    // pass runs one thread per column, o * n / n1 threads, and hides its load latency only with
+    // about 8 waves per CU: the longest n1 that keeps that many, else the shortest.  False if none.
+    bool split_six_step_factors(int64_t n, int64_t o, int64_t &n1, int64_t &m1, int64_t &m2)
+    {
+        if ((plan.flags & kPlanUnalignedRows) || getenv_flag("AFFT_NO_SPLIT_SIX") || getenv_flag("AFFT_NO_ROWS") ||
+            getenv_flag("AFFT_NO_ROW_TR"))
+            return false;
+        const int64_t min_threads = (int64_t)dev.cus * 8 * 64;
+        bool found = false;
+        for (int64_t c : {64, 32, 16}) {
+            if (n % c || !regcol_supported(plan.dtype, c)) continue;
+            const int64_t n2 = n / c;
+            for (int64_t r = 1024; r >= 16; --r) {
+                if (n2 % r) continue;
+                const int tr = row_rows_per_tile(plan.dtype, (int)r, kRowTransposed);
+                const int64_t q = n2 / r;
+                if (tr <= 1 || q < 2 || (c * q) % tr) continue;
+                if (q > dev.max_lds_len[plan.dtype] || !is_5smooth(q) || split_strided(q, r)) continue;
+                n1 = c;
+                m1 = q;
+                m2 = r;
+                found = true;
+                break;
+            }
+            if (found && o * n / n1 >= min_threads) return true;
+        }
+        return found;
+    }
+
+    // Six-step transform of o contiguous rows n = n1 * n2 whose rows n2 = m1 * m2 are themselves
+    // split (element i = i1*n2 + j1*m2 + j2, output k = k1 + n1*(q1 + m1*q2)); three passes that
+    // each read and write whole lines:
+    //   A  columns: length n1 over i1 (stride n2), twiddled by W_n^{i2 k1}, in place (src -> cur)
+    //   B  columns: length m1 over j1 (stride m2) for every (o, k1, j2), twiddled by
+    //      W_n2^{j2 q1}, in place on cur
+    //   C  rows: length m2 over j2 for every (o, q1, k1), taken k1 fastest through a row table and
+    //      stored transposed: X[k1 + n1*q1 + n1*m1*q2] into other(cur)
+    void add_split_six_step(int64_t n, int64_t n1, int64_t m1, int64_t m2, int64_t o, Slot src, Slot cur)
+    {
+        const int64_t n2 = m1 * m2;
+        const void *hi = nullptr, *lo = nullptr, *hi2 = nullptr, *lo2 = nullptr;
+        int sh = 0, sh2 = 0;
+        if (!power_tables(n, hi, lo, sh) || !power_tables(n2, hi2, lo2, sh2)) return;
+        const Slot next = other(cur);
+        int64_t &room = next == Slot::kWork ? plan.work_elems
+                        : next == Slot::kAux2 ? plan.aux2_elems : plan.aux_elems;
+        if (next != Slot::kOut) room = std::max(room, o * n);
+        Addr ia{};
+        ia.cnt0 = 1;
+        ia.cnt1 = n2;
+        ia.st0 = 1;
+        ia.st1 = 1;
+        ia.st2 = n;
+        ia.es = n2;
+        add_stockham(n1, o * n2, ia, ia, true, src, cur, n, 1, hi, lo, sh);
+        if (status) return;
+        Addr ja{};
+        ja.cnt0 = m2;
+        ja.cnt1 = n1;
+        ja.st0 = 1;
+        ja.st1 = n2;
+        ja.st2 = n;
+        ja.es = m2;
+        add_stockham(m1, o * n1 * m2, ja, ja, true, cur, cur, n2, 0, hi2, lo2, sh2);
+        if (status) return;
+        // row r = k1 + n1*(q1 + m1*b) of transform b starts at b*n + k1*n2 + q1*m2
+        const int64_t nrows = o * n1 * m1;
+        std::vector<int64_t> offs((size_t)nrows);
+        for (int64_t r = 0; r < nrows; ++r) {
+            const int64_t k1 = r % n1, q1 = (r / n1) % m1, b = r / (n1 * m1);
+            offs[(size_t)r] = b * n + k1 * n2 + q1 * m2;
+        }
+        void *table = alloc(nrows * (int64_t)sizeof(int64_t));
+        if (status) return;
+        check(hipMemcpyAsync(table, offs.data(), (size_t)nrows * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        check(hipStreamSynchronize(s));
+        if (status) return;
+        add_rows(m2, nrows, cur, next, kRowTransposed);
+        plan.launches.back().ra.tr_n1 = n1 * m1;
+        plan.launches.back().ra.roffs = static_cast<const int64_t *>(table);
+    }
+
     // Transform one axis, reading slot src.  Single-pass kinds run in place or out of place and
     // leave the result in `cur`; a four-step split leaves it in other(cur), since its second pass
     // permutes the elements and cannot run in place.  Returns the slot holding the result.
@@ -640,6 +725,11 @@ struct Builder {
         if (s_ == 1) {
             if (const int64_t n1 = six_step_n1(n)) {
                 add_six_step(n, n1, o, src, cur);
+                return other(cur);
+            }
+            int64_t n1 = 0, m1 = 0, m2 = 0;
+            if (split_six_step_factors(n, o, n1, m1, m2)) {
+                add_split_six_step(n, n1, m1, m2, o, src, cur);
                 return other(cur);
             }
         }

@@ -165,6 +165,26 @@ def test_long_rows_four_step(gpu, dtype, n):
         check(afft.fft(mode, xd), O.fft(OMODE[mode], xr), dtype, n, f"n={n} {mode}")
 
 
+@pytest.mark.parametrize("shape", [(1, 1 << 23), (1, 1 << 22), (1, 1 << 21), (1, 3 << 20), (1, 5 << 19),
+                                   (3, 1 << 20)])
+def test_long_rows_split_six_step(gpu, shape):
+    """Rows whose six-step row length has no compiled row kernel: a register column pass (64, 32
+    or 16 points, by the thread count), a strided LDS pass, and transposed-store rows of 512, 768
+    or 640 read through a row table -- three launches."""
+    n = shape[-1]
+    x = O.uniform(11, shape)
+    xd = dev(x, torch.complex64, gpu)
+    xr = host(xd)
+    p = afft.Plan(shape, torch.complex64)
+    assert p.info()["launches"] == 3
+    for mode in Mode:
+        check(afft.fft(mode, xd), O.fft(OMODE[mode], xr), torch.complex64, n, f"{shape} {mode}")
+    y = xd.clone()
+    p.execute(y, direction=+1, scale=1.0 / n, out=y)  # Inverse, in place
+    check(y, O.fft(O.Mode.Inverse, xr), torch.complex64, n, f"{shape} in place")
+    p.close()
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_bluestein_primes_and_ragged_lengths(gpu, dtype):
     # 4099 (Complex Double) and 10007 pad beyond one workgroup: the multi-kernel path
