@@ -239,32 +239,47 @@ class RaggedPlan:
             pass
 
 
-def spectral_conv(x, H):
-    """y_b = Inverse(fft2D(x_b) * H) for a batch x of shape (B, h, w) and filter H (h, w)."""
+def spectral_conv(x, H, save_spectrum=False):
+    """y_b = Inverse(fft2D(x_b) * H) for a batch x of shape (B, h, w) and filter H (h, w).
+    With save_spectrum, returns (y, F x): the spectrum the VJP can take instead of x."""
     x = _device_tensor(x)
     H = _device_tensor(H)
     if x.dim() != 3 or tuple(H.shape) != tuple(x.shape[1:]) or H.dtype != x.dtype:
         raise ValueError("expected x (B, h, w) and H (h, w) of the same dtype")
     y = torch.empty_like(x)
     with torch.cuda.device(x.device):
+        if save_spectrum:
+            spec = torch.empty_like(x)
+            L.check(L.lib().afft_spectral_conv_fwd_save(_dtype_code(x), x.shape[0], x.shape[1], x.shape[2],
+                                                        _ptr(x), _ptr(H), _ptr(y), _ptr(spec),
+                                                        _stream(x.device)), "afft_spectral_conv_fwd_save")
+            return y, spec
         L.check(L.lib().afft_spectral_conv_fwd(_dtype_code(x), x.shape[0], x.shape[1], x.shape[2],
                                                _ptr(x), _ptr(H), _ptr(y), _stream(x.device)),
                 "afft_spectral_conv_fwd")
     return y
 
 
-def spectral_conv_vjp(x, H, gy, want_x=True, want_H=True):
-    """(gx, gH) for y = spectral_conv(x, H) given dL/dy = gy (conjugate-Wirtinger)."""
-    x = _device_tensor(x)
+def spectral_conv_vjp(x, H, gy, want_x=True, want_H=True, spectrum=None):
+    """(gx, gH) for y = spectral_conv(x, H) given dL/dy = gy (conjugate-Wirtinger).  F x is
+    recomputed from x, or taken from `spectrum` (as spectral_conv(..., save_spectrum=True) returns
+    it; x may then be None)."""
     H = _device_tensor(H)
     gy = _device_tensor(gy)
-    gx = torch.empty_like(x) if want_x else None
+    if spectrum is not None:
+        src = _device_tensor(spectrum)
+        fn, name = L.lib().afft_spectral_conv_vjp_saved, "afft_spectral_conv_vjp_saved"
+    else:
+        src = _device_tensor(x)
+        fn, name = L.lib().afft_spectral_conv_vjp, "afft_spectral_conv_vjp"
+    if src.shape != gy.shape or src.dtype != gy.dtype:
+        raise ValueError("x (or the spectrum) and gy must have the same shape and dtype")
+    gx = torch.empty_like(gy) if want_x else None
     gH = torch.empty_like(H) if want_H else None
-    with torch.cuda.device(x.device):
-        L.check(L.lib().afft_spectral_conv_vjp(
-            _dtype_code(x), x.shape[0], x.shape[1], x.shape[2], _ptr(x), _ptr(H), _ptr(gy),
-            _ptr(gx) if gx is not None else None, _ptr(gH) if gH is not None else None,
-            _stream(x.device)), "afft_spectral_conv_vjp")
+    with torch.cuda.device(gy.device):
+        L.check(fn(_dtype_code(gy), gy.shape[0], gy.shape[1], gy.shape[2], _ptr(src), _ptr(H), _ptr(gy),
+                   _ptr(gx) if gx is not None else None, _ptr(gH) if gH is not None else None,
+                   _stream(gy.device)), name)
     return gx, gH
 
 

@@ -3,7 +3,12 @@
 //
 //   forward   y_b = (1/hw) F^H (H * F x_b)       F = unnormalised 2D DFT, F^H = Reverse mode
 //   VJP       gx_b = (1/hw) F^H (conj(H) * F gy_b)
-//             gH   = (1/hw) sum_b conj(F x_b) * F gy_b      (F x_b recomputed, never stored)
+//             gH   = (1/hw) sum_b conj(F x_b) * F gy_b
+//
+// F x_b is either recomputed by the VJP (afft_spectral_conv_vjp: no memory beyond the call) or
+// stored by the forward call, which has it in registers between its column transform and the
+// filter product (afft_spectral_conv_fwd_save: one more array written), and read back by the
+// VJP (afft_spectral_conv_vjp_saved: no row pass over x, one column transform less per image).
 //
 // Launch structure (forward): row pass over all B*h rows (the persistent row kernel), then ONE
 // column kernel per column tile that does the column FFT, the product with H and the inverse
@@ -28,7 +33,8 @@ namespace afft {
 
 struct ColConvArgs {
     void *data;          // B x h x w, row-transformed; column-transformed result written back
-    const void *other;   // VJP: the row-transformed x (B x h x w)
+    const void *other;   // VJP: the row-transformed x, or F x itself when other_is_spec (B x h x w)
+    void *spec;          // forward: where F x is stored (B x h x w), or null
     const void *H;       // h x w filter
     void *partial;       // VJP: groups x h x w partial sums of conj(X) G
     const void *tw;
@@ -36,6 +42,7 @@ struct ColConvArgs {
     int64_t B, h, w;
     int C;               // columns per tile
     int groups;          // VJP: images are split into `groups` contiguous groups
+    int other_is_spec;   // VJP: `other` holds F x (column transform already done)
 };
 
 template <typename Real, int E>
@@ -58,6 +65,11 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 #pragma unroll
     for (int i = 0; i < E; ++i) reg[i] = active ? d[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
     fft_core<E>(reg, lds + c, C, a.pt, tw, t, T);
+    if (a.spec && active) { // F x for a later VJP
+        V *sp = reinterpret_cast<V *>(a.spec) + b * a.h * a.w + j;
+#pragma unroll
+        for (int i = 0; i < E; ++i) sp[(int64_t)(t + i * T) * a.w] = reg[i];
+    }
     // product with H, then the inverse column transform as conj(F(conj(.)))
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -109,7 +121,7 @@ __global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(
             X[i] = active ? x[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
             G[i] = active ? gy[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
         }
-        fft_core<E>(X, lds + c, C, a.pt, tw, tl, T);
+        if (!a.other_is_spec) fft_core<E>(X, lds + c, C, a.pt, tw, tl, T); // uniform branch
         fft_core<E>(G, lds + c, C, a.pt, tw, tl, T);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
@@ -236,19 +248,19 @@ int col_tables(int dtype, int64_t h, int device, const ColShape &cs, const ColTa
 // Unfused composition for shapes without a fused column layout: batched 2D plans plus pointwise
 // kernels.  Same mathematics, more array passes.
 int conv_fwd_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, const void *H, void *y,
-                     int dev, hipStream_t s)
+                     void *spec, int dev, hipStream_t s)
 {
     const int64_t ext[3] = {B, h, w};
     const Plan *p2 = nullptr;
     int st = cached_plan(p2, dtype, 3, ext, AFFT_AXES_LAST2, dev);
     if (st) return st;
-    if ((st = exec_plan(*p2, -1, x, y, 1.0, s))) return st;
-    if (pointwise_mul(dtype, y, y, H, B * h * w, h * w, 0, 1.0, s) != hipSuccess) return AFFT_ERROR_HIP;
+    if ((st = exec_plan(*p2, -1, x, spec ? spec : y, 1.0, s))) return st;
+    if (pointwise_mul(dtype, y, spec ? spec : y, H, B * h * w, h * w, 0, 1.0, s) != hipSuccess) return AFFT_ERROR_HIP;
     return exec_plan(*p2, +1, y, y, 1.0 / (double)(h * w), s);
 }
 
-int conv_vjp_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, const void *H,
-                     const void *gy, void *gx, void *gH, int dev, hipStream_t s)
+int conv_vjp_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, bool x_is_spec,
+                     const void *H, const void *gy, void *gx, void *gH, int dev, hipStream_t s)
 {
     const int64_t ext[3] = {B, h, w};
     const Plan *p2 = nullptr;
@@ -260,10 +272,11 @@ int conv_vjp_unfused(int dtype, int64_t B, int64_t h, int64_t w, const void *x, 
     if (hipMallocAsync(&ws, 2 * (size_t)n * esz, s) != hipSuccess) return AFFT_ERROR_ALLOC;
     void *X = ws, *G = (char *)ws + (size_t)n * esz;
     const double scale = 1.0 / (double)m;
-    if (!(st = exec_plan(*p2, -1, x, X, 1.0, s))) st = exec_plan(*p2, -1, gy, G, 1.0, s);
+    if (!x_is_spec) st = exec_plan(*p2, -1, x, X, 1.0, s);
+    if (!st) st = exec_plan(*p2, -1, gy, G, 1.0, s);
     if (!st && gH) {
-        // X <- G * conj(X), then the batch sum
-        if (pointwise_mul(dtype, X, G, X, n, n, 1, 1.0, s) != hipSuccess) st = AFFT_ERROR_HIP;
+        // X <- G * conj(F x), then the batch sum
+        if (pointwise_mul(dtype, X, G, x_is_spec ? x : X, n, n, 1, 1.0, s) != hipSuccess) st = AFFT_ERROR_HIP;
         const int64_t g = std::min<int64_t>(4096, (m + 255) / 256);
         if (!st) {
             if (dtype == AFFT_C64)
@@ -295,17 +308,18 @@ int check_args(int dtype, int64_t B, int64_t h, int64_t w)
 
 using namespace afft;
 
-extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
-                                      const void *d_H, void *d_y, void *stream)
+static int conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x, const void *d_H, void *d_y,
+                    void *d_spec, void *stream)
 {
     int st = check_args(dtype, batch, h, w);
     if (st) return st;
     if (batch * h * w == 0) return AFFT_SUCCESS;
     if (!d_x || !d_H || !d_y) return AFFT_ERROR_INVALID_VALUE;
+    if (d_spec && (d_spec == d_y || d_spec == d_H)) return AFFT_ERROR_INVALID_VALUE;
     int dev;
     if ((st = current_device(dev))) return st;
     ColShape cs;
-    if (!col_shape(dtype, h, cs, false)) return conv_fwd_unfused(dtype, batch, h, w, d_x, d_H, d_y, dev, (hipStream_t)stream);
+    if (!col_shape(dtype, h, cs, false)) return conv_fwd_unfused(dtype, batch, h, w, d_x, d_H, d_y, d_spec, dev, (hipStream_t)stream);
     const ColTables *ct = nullptr;
     if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
     const int64_t rows[2] = {batch * h, w};
@@ -317,6 +331,7 @@ extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64
     // 2. column transform, product with H, inverse column transform, in place on y
     ColConvArgs a{};
     a.data = d_y;
+    a.spec = d_spec;
     a.H = d_H;
     a.tw = ct->tw;
     a.pt = ct->pt;
@@ -332,9 +347,22 @@ extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64
     return run_plan(*rp, +1, d_y, d_y, 1.0 / (double)(h * w), stream);
 }
 
-extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
-                                      const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
-                                      void *stream)
+extern "C" int afft_spectral_conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                      const void *d_H, void *d_y, void *stream)
+{
+    return conv_fwd(dtype, batch, h, w, d_x, d_H, d_y, nullptr, stream);
+}
+
+extern "C" int afft_spectral_conv_fwd_save(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                           const void *d_H, void *d_y, void *d_spec, void *stream)
+{
+    if (!d_spec && batch * h * w != 0) return AFFT_ERROR_INVALID_VALUE;
+    return conv_fwd(dtype, batch, h, w, d_x, d_H, d_y, d_spec, stream);
+}
+
+// d_x is x, or F x when x_is_spec
+static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x, bool x_is_spec,
+                    const void *d_H, const void *d_gy, void *d_gx, void *d_gH, void *stream)
 {
     int st = check_args(dtype, batch, h, w);
     if (st) return st;
@@ -342,6 +370,7 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
     hipStream_t s = (hipStream_t)stream;
     if (h * w == 0) return AFFT_SUCCESS;
     if (!d_x || !d_H || !d_gy || (!d_gx && !d_gH)) return AFFT_ERROR_INVALID_VALUE;
+    if (x_is_spec && (d_x == d_gx || d_x == d_gH)) return AFFT_ERROR_INVALID_VALUE;
     if (batch == 0) {
         if (d_gH && hipMemsetAsync(d_gH, 0, (size_t)(h * w) * esz, s) != hipSuccess) return AFFT_ERROR_HIP;
         return AFFT_SUCCESS;
@@ -349,7 +378,7 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
     int dev;
     if ((st = current_device(dev))) return st;
     ColShape cs;
-    if (!col_shape(dtype, h, cs, true)) return conv_vjp_unfused(dtype, batch, h, w, d_x, d_H, d_gy, d_gx, d_gH, dev, s);
+    if (!col_shape(dtype, h, cs, true)) return conv_vjp_unfused(dtype, batch, h, w, d_x, x_is_spec, d_H, d_gy, d_gx, d_gH, dev, s);
     const ColTables *ct = nullptr;
     if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
     const int64_t rows[2] = {batch * h, w};
@@ -363,18 +392,22 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
     const size_t arr = (size_t)(batch * h * w) * esz;
     const size_t part = (size_t)groups * (size_t)(h * w) * esz;
     void *ws = nullptr;
-    // row-transformed x; row-transformed gy lives in gx (or in the workspace when gx is not wanted)
-    const size_t ws_bytes = arr + part + (d_gx ? 0 : arr);
+    // row-transformed x (unless F x is given); row-transformed gy lives in gx (or in the workspace
+    // when gx is not wanted)
+    const size_t xr_bytes = x_is_spec ? 0 : arr;
+    const size_t ws_bytes = xr_bytes + part + (d_gx ? 0 : arr);
     if (hipMallocAsync(&ws, ws_bytes, s) != hipSuccess) return AFFT_ERROR_ALLOC;
     char *base = (char *)ws;
-    void *xr = base;
-    void *partial = base + arr;
-    void *gr = d_gx ? d_gx : (void *)(base + arr + part);
-    if (!(st = run_plan(*rp, -1, d_x, xr, 1.0, stream))) st = run_plan(*rp, -1, d_gy, gr, 1.0, stream);
+    const void *xr = x_is_spec ? d_x : (const void *)base;
+    void *partial = base + xr_bytes;
+    void *gr = d_gx ? d_gx : (void *)(base + xr_bytes + part);
+    if (!x_is_spec) st = run_plan(*rp, -1, d_x, const_cast<void *>(xr), 1.0, stream);
+    if (!st) st = run_plan(*rp, -1, d_gy, gr, 1.0, stream);
     if (!st) {
         ColConvArgs a{};
         a.data = gr;
         a.other = xr;
+        a.other_is_spec = x_is_spec ? 1 : 0;
         a.H = d_H;
         a.partial = partial;
         a.tw = ct->tw;
@@ -404,4 +437,18 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
     if (!st && d_gx) st = run_plan(*rp, +1, d_gx, d_gx, scale, stream);
     (void)hipFreeAsync(ws, s);
     return st;
+}
+
+extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                      const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
+                                      void *stream)
+{
+    return conv_vjp(dtype, batch, h, w, d_x, false, d_H, d_gy, d_gx, d_gH, stream);
+}
+
+extern "C" int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_spec,
+                                            const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
+                                            void *stream)
+{
+    return conv_vjp(dtype, batch, h, w, d_spec, true, d_H, d_gy, d_gx, d_gH, stream);
 }

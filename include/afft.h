@@ -219,6 +219,18 @@ int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const
                            const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
                            void *stream);
 
+/* The forward that also stores F x_b, the unnormalised 2D spectrum of every image (B x h x w), in
+ * d_spec: the array the VJP otherwise recomputes, written from the column kernel's registers (one
+ * more array written).  d_spec must not be d_y or d_H. */
+int afft_spectral_conv_fwd_save(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                const void *d_H, void *d_y, void *d_spec, void *stream);
+
+/* The VJP from a stored spectrum d_spec = F x (as afft_spectral_conv_fwd_save writes it): no row
+ * pass over x and one column transform less per image.  d_spec must not be d_gx or d_gH. */
+int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_spec,
+                                 const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
+                                 void *stream);
+
 /* Autotuning: time the candidate kernel layouts for a key on the device and remember the best in
  * the plan cache; afft_tune_save/afft_tune_load persist the choices as a text file keyed by
  * (gfx arch, HIP runtime version, dtype, length, layout). */

@@ -1,5 +1,7 @@
 """Fused spectral convolution y = Inverse(fft2D(x) * H) and its VJP, against the oracle, against
 the reference's own Reverse mode, and against central finite differences in FP64."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -66,6 +68,45 @@ def test_spectral_conv_vjp_matches_oracle(gpu, dtype, shape):
     assert none is None and O.rel_l2(host(gx2), wx) <= tol
     none, gH2 = afft.spectral_conv_vjp(xd, Hd, gd, want_x=False)
     assert none is None and O.rel_l2(host(gH2), wH) <= tol * 4
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("shape", [(4, 64, 64), (64, 128, 128), (3, 14, 10)])
+def test_spectral_conv_saved_spectrum(gpu, dtype, shape):
+    """The forward that stores F x and the VJP that reads it instead of transforming x (fused
+    shapes, and h = 14, which is not 5-smooth: the unfused composition), against the recomputing
+    forms and the oracle."""
+    B, h, w = shape
+    x, H, gy = O.normal(13, shape), O.normal(14, (h, w)), O.normal(15, shape)
+    xd, Hd, gd = dev(x, dtype, gpu), dev(H, dtype, gpu), dev(gy, dtype, gpu)
+    tol = tolerance(dtype == torch.complex128, h * w)
+    y, spec = afft.spectral_conv(xd, Hd, save_spectrum=True)
+    assert O.rel_l2(host(y), host(afft.spectral_conv(xd, Hd))) <= tol
+    assert O.rel_l2(host(spec), np.stack([O.fft2D(O.Mode.Forward, xb) for xb in host(xd)])) <= tol
+    keep = spec.clone()
+    gx, gH = afft.spectral_conv_vjp(None, Hd, gd, spectrum=spec)
+    assert torch.equal(spec, keep)  # read, not modified
+    wx, wH = oracle_vjp(host(xd), host(Hd), host(gd))
+    assert O.rel_l2(host(gx), wx) <= tol
+    assert O.rel_l2(host(gH), wH) <= tol * 4
+    rx, rH = afft.spectral_conv_vjp(xd, Hd, gd)
+    assert O.rel_l2(host(gx), host(rx)) <= tol and O.rel_l2(host(gH), host(rH)) <= tol * 4
+    gx2, none = afft.spectral_conv_vjp(None, Hd, gd, want_H=False, spectrum=spec)
+    assert none is None and O.rel_l2(host(gx2), wx) <= tol
+    none, gH2 = afft.spectral_conv_vjp(None, Hd, gd, want_x=False, spectrum=spec)
+    assert none is None and O.rel_l2(host(gH2), wH) <= tol * 4
+
+
+def test_spectral_saved_spectrum_rejects_aliasing(gpu):
+    lib = afft._lib.lib()
+    x = torch.zeros(2, 16, 16, dtype=torch.complex64, device=gpu)
+    H = torch.zeros(16, 16, dtype=torch.complex64, device=gpu)
+    y = torch.empty_like(x)
+    s = torch.cuda.current_stream().cuda_stream
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib.afft_spectral_conv_fwd_save(0, 2, 16, 16, p(x), p(H), p(y), p(y), ctypes.c_void_p(s)) == 1
+    assert lib.afft_spectral_conv_fwd_save(0, 2, 16, 16, p(x), p(H), p(y), None, ctypes.c_void_p(s)) == 1
+    assert lib.afft_spectral_conv_vjp_saved(0, 2, 16, 16, p(x), p(H), p(y), p(x), None, ctypes.c_void_p(s)) == 1
 
 
 def test_spectral_conv_vjp_finite_differences_fp64(gpu):
