@@ -519,8 +519,9 @@ struct Builder {
     }
 
     // Six-step factors of a contiguous row of length n (n = n1 * n2): a register-resident column
-    // length n1 whose rows n2 have a compiled row kernel; 0 if none (the balanced four-step,
-    // whose LDS passes beat a six-step with an LDS column pass).
+    // length n1 whose rows n2 have a compiled row kernel; 0 if none (then the split six-step
+    // below, else the balanced four-step; a six-step with an LDS column pass measured slower than
+    // both).
     int64_t six_step_n1(int64_t n)
     {
         if (getenv_flag("AFFT_NO_SIX_STEP")) return 0;
@@ -733,7 +734,8 @@ struct Builder {
                 return other(cur);
             }
         }
-        // four-step: n = n1 * n2, element i = i1*n2 + i2, output k = k1 + n1*k2
+        // balanced four-step (strided axes, and rows neither six-step form covers):
+        // n = n1 * n2, element i = i1*n2 + i2, output k = k1 + n1*k2
         int64_t n1 = 0, n2 = 0;
         double best = 1e300;
         for (int64_t d = 2; d <= limit; ++d) {
