@@ -185,6 +185,29 @@ def test_long_rows_split_six_step(gpu, shape):
     p.close()
 
 
+def _smooth_lengths(lo, hi):
+    out = []
+    for n in range(lo, hi + 1):
+        m = n
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            out.append(n)
+    return out
+
+
+@pytest.mark.parametrize("dtype,stride", [(torch.complex64, 8), (torch.complex128, 24)])
+def test_long_row_lengths_sweep(gpu, dtype, stride):
+    """Every stride-th 5-smooth length in (16384, 2^21], odd ones included: single-workgroup,
+    six-step, split six-step and balanced four-step plans, whichever each length selects."""
+    lengths = _smooth_lengths(16385, 1 << 21)[::stride]
+    for n in lengths:
+        x = O.uniform(n % 1000, (1, n))
+        xd = dev(x, dtype, gpu)
+        check(afft.fft(Mode.Forward, xd), O.fft(O.Mode.Forward, host(xd)), dtype, n, f"n={n}")
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_bluestein_primes_and_ragged_lengths(gpu, dtype):
     # 4099 (Complex Double) and 10007 pad beyond one workgroup: the multi-kernel path
