@@ -480,4 +480,164 @@ __device__ __forceinline__ void fft_core(V (&reg)[E], V *lds_base, int es, const
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// Compile-time Stockham helpers (the row kernels of afft_rows.hip and the column kernels of
+// afft_spectral.hip): workgroup LDS barrier, radix lists, a compile-time loop, and the twiddle
+// powers of one butterfly formed from two loaded bases.
+
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int... Rs> struct Radices {
+    static constexpr int NP = sizeof...(Rs);
+    static constexpr int R[NP] = {Rs...};
+    static constexpr int ns(int p)
+    {
+        int v = 1;
+        for (int i = 0; i < p; ++i) v *= R[i];
+        return v;
+    }
+};
+
+template <int P> struct IC {
+    static constexpr int value = P;
+};
+
+template <typename F, int... Ps>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Ps...>)
+{
+    (f(IC<Ps>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// w[r] = W^{r k} for r = 1..R-1 from b1 = W^k and b4 = W^{4k} (b4 used when R > 4).
+template <int R, typename V> __device__ __forceinline__ void derive_twiddles(V *w, V b1, V b4)
+{
+    w[1] = b1;
+    if constexpr (R > 2) w[2] = c_mul(b1, b1);
+    if constexpr (R > 3) w[3] = c_mul(w[2], b1);
+    if constexpr (R == 5) w[4] = c_mul(w[2], w[2]);
+    if constexpr (R > 5) {
+        w[4] = b4;
+#pragma unroll
+        for (int r = 5; r < R; ++r) {
+            const int q = r & ~3, m = r & 3;
+            if (m == 0) w[r] = c_mul(w[q - 4], b4);
+            else w[r] = c_mul(w[q], w[m]);
+        }
+    }
+}
+
+// v[r] *= W^{r k} for r = 1..R-1, the powers formed on the fly from b1 = W^k, b4 = W^{4k}
+// (the same products as derive_twiddles, with only the bases w1..w3, w4, w8, w12 kept live).
+template <int R, typename V> __device__ __forceinline__ void apply_twiddles(V *v, V b1, V b4)
+{
+    v[1] = c_mul(v[1], b1);
+    if constexpr (R > 2) {
+        const V w2 = c_mul(b1, b1);
+        v[2] = c_mul(v[2], w2);
+        if constexpr (R > 3) {
+            const V w3 = c_mul(w2, b1);
+            v[3] = c_mul(v[3], w3);
+            if constexpr (R == 5) v[4] = c_mul(v[4], c_mul(w2, w2));
+            if constexpr (R > 5) {
+                V w4 = b4;
+#pragma unroll
+                for (int q = 4; q < R; q += 4) {
+                    if (q > 4) w4 = c_mul(w4, b4); // W^{q k}
+                    v[q] = c_mul(v[q], w4);
+                    v[q + 1] = c_mul(v[q + 1], c_mul(w4, b1));
+                    v[q + 2] = c_mul(v[q + 2], c_mul(w4, w2));
+                    v[q + 3] = c_mul(v[q + 3], c_mul(w4, w3));
+                }
+            }
+        }
+    }
+}
+
+// A transform of length N = R0 R1 ... with E elements per thread in natural layout (reg[i] =
+// element t + i T, T = N / E) whose exchanges go through LDS at base[e * es]: the Stockham passes
+// of afft_rows, with every radix, stride and butterfly index a compile-time constant and the
+// twiddle bases (W^k and W^4k of each pass, from the table W_N^m) loaded once by load().  The
+// exchanges synchronise the whole workgroup, so the T threads of a transform may sit in any
+// waves (column tiles interleave the transforms across lanes).
+template <typename V, int N, int E, int... Rs> struct CtStockham {
+    using RL = Radices<Rs...>;
+    static constexpr int T = N / E;
+    static constexpr int NP = RL::NP;
+    static_assert(T * E == N, "E must divide N");
+    V tb[NP > 0 ? NP : 1][2 * E];
+
+    __device__ __forceinline__ void load(const V *__restrict__ tw, int t)
+    {
+        static_for<NP>([&](auto pc) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int R = RL::R[p];
+            constexpr int Ns = RL::ns(p);
+            if constexpr (Ns > 1) {
+                constexpr int B = E / R;
+                constexpr int NB = (T % Ns == 0 || Ns * R == N) ? 1 : B;
+                constexpr int STEP = N / (Ns * R);
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) {
+                    const int k = (t + jb * T) % Ns;
+                    tb[p][2 * jb] = tw[k * STEP];
+                    tb[p][2 * jb + 1] = (R > 5) ? tw[4 * k * STEP] : tb[p][2 * jb];
+                }
+            }
+        });
+    }
+
+    __device__ __forceinline__ void run(V (&reg)[E], V *base, int es, int t) const
+    {
+        static_for<NP>([&](auto pc) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int R = RL::R[p];
+            constexpr int Ns = RL::ns(p);
+            constexpr int B = E / R;
+            static_for<B>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                V v[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = reg[j + r * B];
+                if constexpr (Ns > 1) {
+                    if constexpr (T % Ns == 0) {
+                        apply_twiddles<R>(v, tb[p][0], tb[p][1]);
+                    } else if constexpr (Ns * R == N) {
+                        const V b1 = twiddle_const<N, j * T>(tb[p][0]);
+                        const V b4 = R > 5 ? twiddle_const<N, 4 * j * T>(tb[p][1]) : b1;
+                        apply_twiddles<R>(v, b1, b4);
+                    } else {
+                        apply_twiddles<R>(v, tb[p][2 * j], tb[p][2 * j + 1]);
+                    }
+                }
+                dft<R>::run(v);
+#pragma unroll
+                for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
+            });
+            if constexpr (p + 1 < NP) {
+                lds_sync(); // the previous exchange has been read by every thread
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    const int b = t + j * T;
+                    const int k = b % Ns;
+                    const int dst = (b - k) * R + k;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) base[(dst + r * Ns) * es] = reg[j + r * B];
+                }
+                lds_sync();
+#pragma unroll
+                for (int i = 0; i < E; ++i) reg[i] = base[(t + i * T) * es];
+            }
+        });
+    }
+};
+
 } // namespace afft

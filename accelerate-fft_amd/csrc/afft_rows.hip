@@ -34,13 +34,6 @@
 
 namespace afft {
 
-__device__ __forceinline__ void lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Synchronisation of an LDS exchange among the T threads of one row.  When they all sit in one
 // wavefront (T divides 64; a row's threads are contiguous), the exchange is private to that
 // wavefront, whose LDS instructions execute in order: ordering the accesses at wavefront scope
@@ -49,76 +42,6 @@ template <int T> __device__ __forceinline__ void row_sync()
 {
     if constexpr (T <= 64 && 64 % T == 0) wave_sync();
     else lds_sync();
-}
-
-template <int... Rs> struct Radices {
-    static constexpr int NP = sizeof...(Rs);
-    static constexpr int R[NP] = {Rs...};
-    static constexpr int ns(int p)
-    {
-        int v = 1;
-        for (int i = 0; i < p; ++i) v *= R[i];
-        return v;
-    }
-};
-
-template <int P> struct IC {
-    static constexpr int value = P;
-};
-
-template <typename F, int... Ps>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Ps...>)
-{
-    (f(IC<Ps>{}), ...);
-}
-template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-// w[r] = W^{r k} for r = 1..R-1 from b1 = W^k and b4 = W^{4k} (b4 used when R > 4).
-template <int R, typename V> __device__ __forceinline__ void derive_twiddles(V *w, V b1, V b4)
-{
-    w[1] = b1;
-    if constexpr (R > 2) w[2] = c_mul(b1, b1);
-    if constexpr (R > 3) w[3] = c_mul(w[2], b1);
-    if constexpr (R == 5) w[4] = c_mul(w[2], w[2]);
-    if constexpr (R > 5) {
-        w[4] = b4;
-#pragma unroll
-        for (int r = 5; r < R; ++r) {
-            const int q = r & ~3, m = r & 3;
-            if (m == 0) w[r] = c_mul(w[q - 4], b4);
-            else w[r] = c_mul(w[q], w[m]);
-        }
-    }
-}
-
-// v[r] *= W^{r k} for r = 1..R-1, the powers formed on the fly from b1 = W^k, b4 = W^{4k}
-// (the same products as derive_twiddles, with only the bases w1..w3, w4, w8, w12 kept live).
-template <int R, typename V> __device__ __forceinline__ void apply_twiddles(V *v, V b1, V b4)
-{
-    v[1] = c_mul(v[1], b1);
-    if constexpr (R > 2) {
-        const V w2 = c_mul(b1, b1);
-        v[2] = c_mul(v[2], w2);
-        if constexpr (R > 3) {
-            const V w3 = c_mul(w2, b1);
-            v[3] = c_mul(v[3], w3);
-            if constexpr (R == 5) v[4] = c_mul(v[4], c_mul(w2, w2));
-            if constexpr (R > 5) {
-                V w4 = b4;
-#pragma unroll
-                for (int q = 4; q < R; q += 4) {
-                    if (q > 4) w4 = c_mul(w4, b4); // W^{q k}
-                    v[q] = c_mul(v[q], w4);
-                    v[q + 1] = c_mul(v[q + 1], c_mul(w4, b1));
-                    v[q + 2] = c_mul(v[q + 2], c_mul(w4, w2));
-                    v[q + 3] = c_mul(v[q + 3], c_mul(w4, w3));
-                }
-            }
-        }
-    }
 }
 
 template <int N, int E> struct RowShape {

@@ -39,19 +39,43 @@ struct ColConvArgs {
     void *partial;       // VJP: groups x h x w partial sums of conj(X) G
     const void *tw;
     PassTable pt;        // length h
+    const void *pw;      // W_h^m, m in [0, h): the twiddle bases of the compile-time transforms
     int64_t B, h, w;
     int C;               // columns per tile
     int groups;          // VJP: images are split into `groups` contiguous groups
     int other_is_spec;   // VJP: `other` holds F x (column transform already done)
 };
 
-template <typename Real, int E>
+// The column transform of a tile: h = N with compile-time radices Rs (the Stockham passes of the
+// row kernels, twiddle bases loaded once per workgroup), or, for N = 0, the generic transform
+// of any 5-smooth h driven by the pass table.
+template <typename V, int E, int N, int... Rs> struct ColFft {
+    CtStockham<V, N, E, Rs...> ct;
+    __device__ __forceinline__ void init(const ColConvArgs &a, int t) { ct.load(reinterpret_cast<const V *>(a.pw), t); }
+    // the thread index passes through an empty asm, so each transform recomputes its LDS
+    // addresses instead of the compiler keeping all of them live across both transforms
+    __device__ __forceinline__ void run(V (&reg)[E], V *base, const ColConvArgs &a, int t) const
+    {
+        int tl = t;
+        __asm__ volatile("" : "+v"(tl));
+        ct.run(reg, base, a.C, tl);
+    }
+};
+template <typename V, int E> struct ColFft<V, E, 0> {
+    __device__ __forceinline__ void init(const ColConvArgs &, int) {}
+    __device__ __forceinline__ void run(V (&reg)[E], V *base, const ColConvArgs &a, int t) const
+    {
+        fft_core<E>(reg, base, a.C, a.pt, reinterpret_cast<const V *>(a.tw), t, (int)(a.h / E));
+    }
+};
+
+template <typename Real, int E, int N, int... Rs>
 __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_colconv(const ColConvArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
-    const int T = (int)(a.h / E), C = a.C;
+    const int T = N ? N / E : (int)(a.h / E), C = a.C;
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int64_t tiles = (a.w + C - 1) / C;
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -60,11 +84,12 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     const bool active = j < a.w && b < a.B;
     V *d = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
     const V *Hc = reinterpret_cast<const V *>(a.H) + j;
-    const V *tw = reinterpret_cast<const V *>(a.tw);
+    ColFft<V, E, N, Rs...> fft;
+    fft.init(a, t);
     V reg[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) reg[i] = active ? d[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
-    fft_core<E>(reg, lds + c, C, a.pt, tw, t, T);
+    fft.run(reg, lds + c, a, t);
     if (a.spec && active) { // F x for a later VJP
         V *sp = reinterpret_cast<V *>(a.spec) + b * a.h * a.w + j;
 #pragma unroll
@@ -76,7 +101,7 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         const V h = active ? Hc[(int64_t)(t + i * T) * a.w] : c_make<V>(0, 0);
         reg[i] = c_conj(c_mul(reg[i], h));
     }
-    fft_core<E>(reg, lds + c, C, a.pt, tw, t, T);
+    fft.run(reg, lds + c, a, t);
     if (active) {
 #pragma unroll
         for (int i = 0; i < E; ++i) d[(int64_t)(t + i * T) * a.w] = c_conj(reg[i]);
@@ -87,13 +112,13 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 // for workgroups of at most kVjpBlock threads so the register budget covers them.
 constexpr int kVjpBlock = 256;
 
-template <typename Real, int E>
+template <typename Real, int E, int N, int... Rs>
 __global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1, 8))) afft_colconv_vjp(const ColConvArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
-    const int T = (int)(a.h / E), C = a.C;
+    const int T = N ? N / E : (int)(a.h / E), C = a.C;
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int64_t tiles = (a.w + C - 1) / C;
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -103,7 +128,8 @@ __global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(
     const int64_t per = (a.B + a.groups - 1) / a.groups;
     const int64_t b0 = g * per, b1 = (b0 + per < a.B) ? b0 + per : a.B;
     const V *Hc = reinterpret_cast<const V *>(a.H) + j;
-    const V *tw = reinterpret_cast<const V *>(a.tw);
+    ColFft<V, E, N, Rs...> fft;
+    fft.init(a, t);
     V acc[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = c_make<V>(0, 0);
@@ -121,15 +147,15 @@ __global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(
             X[i] = active ? x[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
             G[i] = active ? gy[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
         }
-        if (!a.other_is_spec) fft_core<E>(X, lds + c, C, a.pt, tw, tl, T); // uniform branch
-        fft_core<E>(G, lds + c, C, a.pt, tw, tl, T);
+        if (!a.other_is_spec) fft.run(X, lds + c, a, tl); // uniform branch
+        fft.run(G, lds + c, a, tl);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             acc[i] = c_add(acc[i], c_mulc(G[i], X[i])); // conj(X) G
             const V h = active ? Hc[(int64_t)(tl + i * T) * a.w] : c_make<V>(0, 0);
             G[i] = c_conj(c_mulc(G[i], h));             // conj(conj(H) G)
         }
-        fft_core<E>(G, lds + c, C, a.pt, tw, tl, T);
+        fft.run(G, lds + c, a, tl);
         if (active) {
 #pragma unroll
             for (int i = 0; i < E; ++i) gy[(int64_t)(tl + i * T) * a.w] = c_conj(G[i]);
@@ -159,21 +185,33 @@ namespace {
 
 #define AFFT_SPEC_E(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16)
 
+template <typename Real> const void *colconv_fn(bool vjp, int E, int64_t h)
+{
+#define AFFT_COLCONV_CT(H, EV, ...)                                                               \
+    if (h == H && E == EV)                                                                       \
+        return vjp ? reinterpret_cast<const void *>(afft_colconv_vjp<Real, EV, H, __VA_ARGS__>)  \
+                   : reinterpret_cast<const void *>(afft_colconv<Real, EV, H, __VA_ARGS__>);
+    // compile-time transforms for the common heights (the radices of the row kernels)
+    AFFT_COLCONV_CT(256, 16, 16, 16)
+    AFFT_COLCONV_CT(512, 16, 16, 16, 2)
+#undef AFFT_COLCONV_CT
+    switch (E) {
+#define X(EV)                                                                                    \
+    case EV:                                                                                     \
+        return vjp ? reinterpret_cast<const void *>(afft_colconv_vjp<Real, EV, 0>)              \
+                   : reinterpret_cast<const void *>(afft_colconv<Real, EV, 0>);
+        AFFT_SPEC_E(X)
+#undef X
+    default: return nullptr;
+    }
+}
+
 template <typename Real>
 hipError_t launch_colconv(bool vjp, int E, const ColConvArgs &a, int block, int64_t grid, size_t lds,
                           hipStream_t s)
 {
-    const void *fn = nullptr;
-    switch (E) {
-#define X(EV)                                                                                    \
-    case EV:                                                                                     \
-        fn = vjp ? reinterpret_cast<const void *>(afft_colconv_vjp<Real, EV>)                   \
-                 : reinterpret_cast<const void *>(afft_colconv<Real, EV>);                      \
-        break;
-        AFFT_SPEC_E(X)
-#undef X
-    default: return hipErrorInvalidValue;
-    }
+    const void *fn = colconv_fn<Real>(vjp, E, a.h);
+    if (!fn) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -208,6 +246,7 @@ struct ColTables {
     Plan owner;
     PassTable pt;
     const void *tw = nullptr;
+    const void *pw = nullptr; // W_h^m, m in [0, h)
 };
 
 int col_tables(int dtype, int64_t h, int device, const ColShape &cs, const ColTables *&out)
@@ -233,6 +272,11 @@ int col_tables(int dtype, int64_t h, int device, const ColShape &cs, const ColTa
         return AFFT_ERROR_HIP;
     }
     st = table_pass_twiddles(ct->owner, dev, s, h, cs.L, ct->pt, ct->tw);
+    if (!st) {
+        void *pw = plan_alloc(ct->owner, dev, h * (int64_t)(dtype == AFFT_C64 ? 8 : 16), st);
+        if (!st && gen_power_table(dtype, pw, h, 1, h, s) != hipSuccess) st = AFFT_ERROR_HIP;
+        ct->pw = pw;
+    }
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
     if (st) {
@@ -335,6 +379,7 @@ static int conv_fwd(int dtype, int64_t batch, int64_t h, int64_t w, const void *
     a.H = d_H;
     a.tw = ct->tw;
     a.pt = ct->pt;
+    a.pw = ct->pw;
     a.B = batch;
     a.h = h;
     a.w = w;
@@ -412,6 +457,7 @@ static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *
         a.partial = partial;
         a.tw = ct->tw;
         a.pt = ct->pt;
+        a.pw = ct->pw;
         a.B = batch;
         a.h = h;
         a.w = w;

@@ -35,7 +35,7 @@ def oracle_vjp(x, H, gy):
 
 
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-@pytest.mark.parametrize("shape", [(3, 64, 96), (2, 512, 512), (5, 30, 1), (1, 1, 8), (4, 96, 64)])
+@pytest.mark.parametrize("shape", [(3, 64, 96), (2, 512, 512), (5, 30, 1), (1, 1, 8), (4, 96, 64), (2, 256, 48)])
 def test_spectral_conv_forward(gpu, dtype, shape):
     B, h, w = shape
     x = O.normal(1, shape)
@@ -53,7 +53,8 @@ def test_spectral_conv_forward(gpu, dtype, shape):
 
 
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-@pytest.mark.parametrize("shape", [(4, 64, 64), (7, 32, 48), (64, 128, 128)])
+# h = 256 and 512 take the compile-time column transforms, the others the generic one
+@pytest.mark.parametrize("shape", [(4, 64, 64), (7, 32, 48), (64, 128, 128), (3, 256, 40), (2, 512, 24)])
 def test_spectral_conv_vjp_matches_oracle(gpu, dtype, shape):
     B, h, w = shape
     x, H, gy = O.normal(3, shape), O.normal(4, (h, w)), O.normal(5, shape)
@@ -71,7 +72,7 @@ def test_spectral_conv_vjp_matches_oracle(gpu, dtype, shape):
 
 
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-@pytest.mark.parametrize("shape", [(4, 64, 64), (64, 128, 128), (3, 14, 10)])
+@pytest.mark.parametrize("shape", [(4, 64, 64), (64, 128, 128), (3, 14, 10), (2, 512, 24)])
 def test_spectral_conv_saved_spectrum(gpu, dtype, shape):
     """The forward that stores F x and the VJP that reads it instead of transforming x (fused
     shapes, and h = 14, which is not 5-smooth: the unfused composition), against the recomputing
