@@ -89,6 +89,7 @@ struct FftArgs {
     const void *in;
     void *out;
     const void *tw;      // pass twiddles (element type), pass p at tw + pt.tw_off[p]
+    const void *pw;      // W_n^m, m in [0, n), for the compile-time strided transforms, or null
     int64_t ntrans;
     Addr ain, aout;
     PassTable pt;

@@ -41,19 +41,23 @@ __device__ __forceinline__ V *lds_slot(V *lds, int c, int C, int n, int interlea
 }
 
 // ----------------------------------------------------------------------------------------------
-// Batched Stockham transform.
+// Batched Stockham transform.  N = 0: the generic transform of any supported n, driven by the pass
+// table; N > 0: n = N with compile-time radices Rs (CtStockham, the row kernels' passes; used for
+// the interleaved tiles of strided axes), twiddle bases loaded once from the table W_N^m.
 
-template <typename Real, int E>
+template <typename Real, int E, int N, int... Rs>
 __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_stockham(const FftArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
 
-    const int n = a.pt.n;
+    const int n = N ? N : a.pt.n;
     const int T = n / E;
     int c, t;
     split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
+    CtStockham<V, N ? N : E, E, Rs...> ct;
+    if constexpr (N != 0) ct.load(reinterpret_cast<const V *>(a.pw), t);
     const int64_t tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     const int64_t j = tile * a.C + c;
     const bool active = (c < a.C) && (j < a.ntrans);
@@ -79,7 +83,8 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 
     int es;
     V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, n, a.lds_interleave, es);
-    fft_core<E>(reg, slot, es, a.pt, reinterpret_cast<const V *>(a.tw), t, T);
+    if constexpr (N != 0) ct.run(reg, slot, es, t);
+    else fft_core<E>(reg, slot, es, a.pt, reinterpret_cast<const V *>(a.tw), t, T);
 
     if (active) {
         V *out = reinterpret_cast<V *>(a.out);
@@ -334,9 +339,19 @@ static hipError_t launch_one(Kern k, const Args &a, int block, int64_t grid, siz
 template <typename Real>
 static hipError_t launch_stockham_t(const FftArgs &a, int block, int64_t grid, size_t lds, hipStream_t s)
 {
+    if constexpr (sizeof(Real) == 4) {
+        // compile-time transforms for the interleaved tiles of strided axes (the radices the
+        // layout chooser picks for these lengths)
+        if (a.pw && a.lds_interleave && a.E == 16) {
+            if (a.pt.n == 128) return launch_one(afft_stockham<Real, 16, 128, 16, 8>, a, block, grid, lds, s);
+            if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, 16, 16>, a, block, grid, lds, s);
+            if (a.pt.n == 512) return launch_one(afft_stockham<Real, 16, 512, 16, 16, 2>, a, block, grid, lds, s);
+            if (a.pt.n == 1024) return launch_one(afft_stockham<Real, 16, 1024, 16, 16, 4>, a, block, grid, lds, s);
+        }
+    }
     switch (a.E) {
 #define X(EV)                                                                                    \
-    case EV: return launch_one(afft_stockham<Real, EV>, a, block, grid, lds, s);
+    case EV: return launch_one(afft_stockham<Real, EV, 0>, a, block, grid, lds, s);
         AFFT_FOR_EACH_E(X)
 #undef X
     default: return hipErrorInvalidValue;

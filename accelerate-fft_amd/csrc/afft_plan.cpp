@@ -326,6 +326,13 @@ struct Builder {
         a.post_hi = post_hi;
         a.post_lo = post_lo;
         a.post_shift = post_shift;
+        // Complex Float strided tiles of 128 ... 1024 run compile-time passes (the launcher picks
+        // them when the power table is present)
+        if (strided && plan.dtype == AFFT_C64 && L.E == 16 && (n == 128 || n == 256 || n == 512 || n == 1024) &&
+            !getenv_flag("AFFT_NO_CT_COLS")) {
+            a.pw = power_table(n);
+            if (status) return;
+        }
         l.grid = (ntrans + a.C - 1) / a.C;
         plan.launches.push_back(l);
     }
