@@ -15,8 +15,8 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
              copy kernel of known size), read from profiles/ when present.
   cpu_baseline  the oracle's recursive FFT in float arithmetic (the CPU path restated; FFTW is
              not installed), over the same 16384 x 4096 array, on every host core available.
-  extras     the other configurations (four-step 2^22 row, Reverse/Inverse, fft2D 4096^2,
-             fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
+  extras     the other configurations (four-step 2^22 row, Reverse/Inverse, fft2D 4096^2, batched
+             fft2D 64 x 512^2, fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
              time and roofline fraction; the ragged batch and the spectral convolution also
              against the plain per-call path on the same GPU.
 
@@ -391,6 +391,12 @@ def main():
         add("fft2D_4096x4096_c64_Forward", lambda: afft.fft2D(Mode.Forward, m2), N, 16.0 * N, 5.0 * N * 24)
         add("fft2D_4096x4096_c64_Inverse", lambda: afft.fft2D(Mode.Inverse, m2), N, 16.0 * N, 5.0 * N * 24)
         del m2
+        # batched 2D (the spectral-convolution batch): 64 images of 512 x 512, two passes
+        mb = normal_c((64, 512, 512), torch.complex64, 5, dev)
+        N = 64 * 512 * 512
+        add("fft2D_many_64x512x512_c64_Forward", lambda: afft.fft2D_many(Mode.Forward, mb), N, 16.0 * N,
+            5.0 * N * 18)
+        del mb
         c3 = normal_c((256, 256, 256), torch.complex128, 3, dev)
         N = 256 ** 3
         add("fft3D_256x256x256_c128_Forward", lambda: afft.fft3D(Mode.Forward, c3), N, 32.0 * N, 5.0 * N * 24)
