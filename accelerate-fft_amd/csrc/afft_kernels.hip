@@ -339,15 +339,15 @@ static hipError_t launch_one(Kern k, const Args &a, int block, int64_t grid, siz
 template <typename Real>
 static hipError_t launch_stockham_t(const FftArgs &a, int block, int64_t grid, size_t lds, hipStream_t s)
 {
-    if constexpr (sizeof(Real) == 4) {
-        // compile-time transforms for the interleaved tiles of strided axes (the radices the
-        // layout chooser picks for these lengths)
-        if (a.pw && a.lds_interleave && a.E == 16) {
+    // compile-time transforms for the interleaved tiles of strided axes (the radices the layout
+    // chooser picks for these lengths)
+    if (a.pw && a.lds_interleave && a.E == 16) {
+        if constexpr (sizeof(Real) == 4) {
             if (a.pt.n == 128) return launch_one(afft_stockham<Real, 16, 128, 16, 8>, a, block, grid, lds, s);
-            if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, 16, 16>, a, block, grid, lds, s);
             if (a.pt.n == 512) return launch_one(afft_stockham<Real, 16, 512, 16, 16, 2>, a, block, grid, lds, s);
             if (a.pt.n == 1024) return launch_one(afft_stockham<Real, 16, 1024, 16, 16, 4>, a, block, grid, lds, s);
         }
+        if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, 16, 16>, a, block, grid, lds, s);
     }
     switch (a.E) {
 #define X(EV)                                                                                    \

@@ -326,10 +326,10 @@ struct Builder {
         a.post_hi = post_hi;
         a.post_lo = post_lo;
         a.post_shift = post_shift;
-        // Complex Float strided tiles of 128 ... 1024 run compile-time passes (the launcher picks
-        // them when the power table is present)
-        if (strided && plan.dtype == AFFT_C64 && L.E == 16 && (n == 128 || n == 256 || n == 512 || n == 1024) &&
-            !getenv_flag("AFFT_NO_CT_COLS")) {
+        // strided tiles of 128 ... 1024 (Complex Float) and 256 (Complex Double) run compile-time
+        // passes (the launcher picks them when the power table is present)
+        const bool ct = plan.dtype == AFFT_C64 ? (n == 128 || n == 256 || n == 512 || n == 1024) : n == 256;
+        if (strided && ct && L.E == 16 && !getenv_flag("AFFT_NO_CT_COLS")) {
             a.pw = power_table(n);
             if (status) return;
         }
