@@ -570,6 +570,7 @@ template <int R, typename V> __device__ __forceinline__ void apply_twiddles(V *v
 // waves (column tiles interleave the transforms across lanes).
 template <typename V, int N, int E, int... Rs> struct CtStockham {
     using RL = Radices<Rs...>;
+    template <bool PAD> static __device__ __forceinline__ int pos(int p) { return PAD ? lds_pad(p) : p; }
     static constexpr int T = N / E;
     static constexpr int NP = RL::NP;
     static_assert(T * E == N, "E must divide N");
@@ -595,7 +596,9 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
         });
     }
 
-    __device__ __forceinline__ void run(V (&reg)[E], V *base, int es, int t) const
+    // PAD: positions pass through lds_pad (a private contiguous slot, es = 1); interleaved tiles
+    // (es = C) measured the same without it
+    template <bool PAD = false> __device__ __forceinline__ void run(V (&reg)[E], V *base, int es, int t) const
     {
         static_for<NP>([&](auto pc) {
             constexpr int p = decltype(pc)::value;
@@ -630,11 +633,11 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
                     const int k = b % Ns;
                     const int dst = (b - k) * R + k;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) base[(dst + r * Ns) * es] = reg[j + r * B];
+                    for (int r = 0; r < R; ++r) base[pos<PAD>(dst + r * Ns) * es] = reg[j + r * B];
                 }
                 lds_sync();
 #pragma unroll
-                for (int i = 0; i < E; ++i) reg[i] = base[(t + i * T) * es];
+                for (int i = 0; i < E; ++i) reg[i] = base[pos<PAD>(t + i * T) * es];
             }
         });
     }

@@ -118,6 +118,7 @@ struct BluesteinArgs {
     const void *in;
     void *out;
     const void *tw;      // pass twiddles for length M
+    const void *pw;      // W_M^m, m in [0, M), for the compile-time transforms of M, or null
     const void *chirp;   // c[m] = exp(-i pi m^2 / n), m < n
     const void *filt;    // FFT_M(b) / M, b[m] = conj(c[|m|]) wrapped
     int64_t ntrans;

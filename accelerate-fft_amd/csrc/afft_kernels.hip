@@ -113,19 +113,22 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 
 // ----------------------------------------------------------------------------------------------
 // Fused Bluestein: X[k] = c[k] * IFFT_M( FFT_M(x c) * FFT_M(b)/M )[k],  c[m] = exp(-i pi m^2/n).
-// The inverse FFT is conj(FFT(conj(.))) so both transforms use the forward passes.
+// The inverse FFT is conj(FFT(conj(.))) so both transforms use the forward passes.  MC = 0: the
+// transforms of M driven by the pass table; MC > 0: M = MC with compile-time radices (CtStockham).
 
-template <typename Real, int E>
+template <typename Real, int E, int MC, int... Rs>
 __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_bluestein(const BluesteinArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
 
-    const int M = a.pt.n;
+    const int M = MC ? MC : a.pt.n;
     const int T = M / E;
     int c, t;
     split_tid(threadIdx.x, a.C, T, a.lane_col, c, t);
+    CtStockham<V, MC ? MC : E, E, Rs...> ct;
+    if constexpr (MC != 0) ct.load(reinterpret_cast<const V *>(a.pw), t);
     const int64_t tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     const int64_t j = tile * a.C + c;
     const bool active = (c < a.C) && (j < a.ntrans);
@@ -156,10 +159,26 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 
     int es;
     V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, M, a.lds_interleave, es);
-    fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+    if constexpr (MC != 0) {
+        // padded positions in a private slot; the thread index passes through an empty asm before
+        // each transform so that the LDS addresses are not all kept live across both
+        int tl = t;
+        __asm__ volatile("" : "+v"(tl));
+        if (a.lds_interleave) ct.run(reg, slot, es, tl);
+        else ct.template run<true>(reg, slot, es, tl);
+    } else {
+        fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
-    fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+    if constexpr (MC != 0) {
+        int tl = t;
+        __asm__ volatile("" : "+v"(tl));
+        if (a.lds_interleave) ct.run(reg, slot, es, tl);
+        else ct.template run<true>(reg, slot, es, tl);
+    } else {
+        fft_core<E>(reg, slot, es, a.pt, tw, t, T);
+    }
 
     if (active) {
         V *out = reinterpret_cast<V *>(a.out);
@@ -362,9 +381,17 @@ template <typename Real>
 static hipError_t launch_bluestein_t(const BluesteinArgs &a, int block, int64_t grid, size_t lds,
                                      hipStream_t s)
 {
+    if constexpr (sizeof(Real) == 4) {
+        // compile-time transforms for the padded lengths of two common primes (257 -> 640,
+        // 4099 -> 10240), with the radices the layout chooser picks
+        if (a.pw) {
+            if (a.pt.n == 640 && a.E == 40) return launch_one(afft_bluestein<Real, 40, 640, 8, 8, 5, 2>, a, block, grid, lds, s);
+            if (a.pt.n == 10240 && a.E == 40) return launch_one(afft_bluestein<Real, 40, 10240, 8, 8, 8, 5, 4>, a, block, grid, lds, s);
+        }
+    }
     switch (a.E) {
 #define X(EV)                                                                                    \
-    case EV: return launch_one(afft_bluestein<Real, EV>, a, block, grid, lds, s);
+    case EV: return launch_one(afft_bluestein<Real, EV, 0>, a, block, grid, lds, s);
         AFFT_FOR_EACH_E(X)
 #undef X
     default: return hipErrorInvalidValue;

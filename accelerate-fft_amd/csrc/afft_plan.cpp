@@ -447,6 +447,11 @@ struct Builder {
         a.lds_interleave = strided ? 1 : 0;
         a.xcd_remap = strided && !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
         a.scale = 1.0;
+        // Complex Float padded lengths 640 and 10240 run compile-time transforms
+        if (plan.dtype == AFFT_C64 && (M == 640 || M == 10240) && !getenv_flag("AFFT_NO_CT_COLS")) {
+            a.pw = power_table(M);
+            if (status) return;
+        }
         l.grid = (ntrans + C - 1) / C;
         plan.launches.push_back(l);
     }
