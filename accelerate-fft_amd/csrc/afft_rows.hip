@@ -114,10 +114,15 @@ afft_rows(const RowArgs a)
         }
     });
 
+    // work item g -> tile.  Transposed stores map the items XCD-contiguously (XCD x, which gets
+    // the workgroups b = x mod 8, takes one contiguous range of tiles): neighbouring tiles write
+    // the two halves of the same output lines, and on one XCD those halves meet in one L2
+    auto tile_of = [&](int64_t item) -> int64_t { return TR ? xcd_tile(item, ntiles) : item; };
     int64_t g = blockIdx.x;
-    if constexpr (PF) load_tile(g);
+    if constexpr (PF) load_tile(tile_of(g));
     for (; g < ntiles; g += gridDim.x) {
-        if constexpr (!PF) load_tile(g);
+        const int64_t gt = tile_of(g);
+        if constexpr (!PF) load_tile(gt);
         // LD: the LDS addresses depend only on the thread; passing its index through an empty
         // asm once per row makes them recomputed here instead of hoisted out of the loop, where
         // with many passes (or a tight register budget) they would keep many registers live
@@ -126,14 +131,14 @@ afft_rows(const RowArgs a)
         V reg[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
-        if (a.sg_in.on && g * C + c < nrows) {
-            const int64_t rb = roffs ? roffs[g * C + c] : (g * C + c) * N;
+        if (a.sg_in.on && gt * C + c < nrows) {
+            const int64_t rb = roffs ? roffs[gt * C + c] : (gt * C + c) * N;
 #pragma unroll
             for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_in, rb + t + i * T, reg[i]);
         }
         if constexpr (PF) {
             __builtin_amdgcn_sched_barrier(0);
-            load_tile(g + gridDim.x);
+            load_tile(tile_of(g + gridDim.x));
             __builtin_amdgcn_sched_barrier(0);
         }
 
@@ -189,7 +194,7 @@ afft_rows(const RowArgs a)
             for (int i = 0; i < E; ++i) lds[(tl + i * T) * C1 + c] = reg[i];
             lds_sync();
             const int c2 = threadIdx.x % C, t2 = threadIdx.x / C;
-            const int64_t row2 = g * C + c2;
+            const int64_t row2 = gt * C + c2;
             if (row2 < nrows) {
                 const int64_t n1 = a.tr_n1;
                 const int64_t o = row2 / n1, k1 = row2 - o * n1;
@@ -206,7 +211,7 @@ afft_rows(const RowArgs a)
             }
             lds_sync(); // the regroup buffer is read before the next tile's first exchange
         } else {
-            const int64_t row = g * C + c;
+            const int64_t row = gt * C + c;
             if (row < nrows && a.sg_out.on) {
                 const int64_t rb = roffs ? roffs[row] : row * N;
 #pragma unroll
