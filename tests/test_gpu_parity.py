@@ -142,7 +142,7 @@ def test_two_split_axes(gpu):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("shape", [(16, 32, 64), (1, 1, 1), (3, 5, 7), (8, 1, 8), (11, 12, 10),
-                                   (32, 32, 32), (64, 16, 48)])
+                                   (32, 32, 32), (64, 16, 48), (256, 4, 8), (4, 256, 8)])
 def test_fft3D(gpu, dtype, shape):
     x = O.normal(11 + sum(shape), shape)
     xd = dev(x, dtype, gpu)
