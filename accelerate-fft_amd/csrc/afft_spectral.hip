@@ -109,11 +109,12 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 }
 
 // Holds three transforms' worth of registers per thread (x, gy and the running sum): compiled
-// for workgroups of at most kVjpBlock threads so the register budget covers them.
-constexpr int kVjpBlock = 256;
+// for workgroups of at most vjp_block threads so the register budget covers them (512 in FP32:
+// 16 columns per tile, whole-line segments, one workgroup of 8 waves per CU).
+constexpr int vjp_block(bool f64) { return f64 ? 256 : 512; }
 
 template <typename Real, int E, int N, int... Rs>
-__global__ void __launch_bounds__(kVjpBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1, 8))) afft_colconv_vjp(const ColConvArgs a)
+__global__ void __launch_bounds__(vjp_block(sizeof(Real) == 8)) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1, 8))) afft_colconv_vjp(const ColConvArgs a)
 {
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
@@ -229,9 +230,9 @@ struct ColShape {
 bool col_shape(int dtype, int64_t h, ColShape &cs, bool vjp)
 {
     if (h > 16384 || !is_5smooth(h)) return false;
-    if (!choose_layout(h, dtype, cs.L, 16, vjp ? kVjpBlock : -1)) return false;
+    if (!choose_layout(h, dtype, cs.L, 16, vjp ? vjp_block(dtype == AFFT_C128) : -1)) return false;
     const int T = (int)(h / cs.L.E);
-    const int maxb = vjp ? kVjpBlock : max_block_threads(dtype == AFFT_C128, cs.L.E);
+    const int maxb = vjp ? vjp_block(dtype == AFFT_C128) : max_block_threads(dtype == AFFT_C128, cs.L.E);
     const size_t esz = dtype == AFFT_C64 ? 8 : 16;
     cs.C = 16;
     while (cs.C > 1 && (cs.C * T > maxb || (size_t)cs.C * lds_padded_len((int)h) * esz > 80 * 1024)) cs.C /= 2;
