@@ -292,8 +292,8 @@ def main():
     ap.add_argument("--profile-workload", choices=sorted(PROFILE_WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path (each GPU would run its own "
+              "shard of rows or images); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
