@@ -244,7 +244,11 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
     if (hipEventRecord(rp.fork, s) != hipSuccess || hipStreamWaitEvent(side, rp.fork, 0) != hipSuccess)
         return AFFT_ERROR_HIP;
     int st = AFFT_SUCCESS;
-    if (rp.nitems > 0) {
+    // the bandwidth-bound long classes on the caller's stream first: they are the critical path,
+    // and enqueued first they start without waiting for the side stream's launches to be issued
+    for (const auto &lp : rp.longs)
+        if (!st && !lp->small) st = run_long(*lp, mode, in, out, s);
+    if (!st && rp.nitems > 0) {
         RaggedArgs a{};
         a.in = in;
         a.out = out;
@@ -258,8 +262,6 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
     }
     for (const auto &lp : rp.longs)
         if (!st && lp->small) st = run_long(*lp, mode, in, out, side);
-    for (const auto &lp : rp.longs)
-        if (!st && !lp->small) st = run_long(*lp, mode, in, out, s);
     // join even after an error, so that the side stream never runs ahead of the caller's
     if (hipEventRecord(rp.join, side) != hipSuccess || hipStreamWaitEvent(s, rp.join, 0) != hipSuccess)
         st = st ? st : AFFT_ERROR_HIP;
