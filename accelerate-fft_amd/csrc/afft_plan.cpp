@@ -248,8 +248,10 @@ struct Builder {
             while (C > 1 && (size_t)C * T > maxb) C /= 2;
             if (npass > 1) {
                 while (C > 1 && (size_t)C * per > dev.lds_bytes) C /= 2;
-                // beyond whole lines, keep room for three workgroups per CU
-                while (C > 16 && (size_t)C * per > dev.lds_bytes / 3) C /= 2;
+                // beyond whole 128-byte lines (16 columns in FP32, 8 in FP64), keep room for three
+                // workgroups per CU
+                const int line_cols = (int)(128 / esz());
+                while (C > line_cols && (size_t)C * per > dev.lds_bytes / 3) C /= 2;
             }
             // prefer at least 64 threads per workgroup when the LDS allows it
         }
