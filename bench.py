@@ -15,7 +15,8 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
              copy kernel of known size), read from profiles/ when present.
   cpu_baseline  the oracle's recursive FFT in float arithmetic (the CPU path restated; FFTW is
              not installed), over the same 16384 x 4096 array, on every host core available.
-  extras     the other configurations (four-step 2^22 row, Reverse/Inverse, fft2D 4096^2, batched
+  extras     the other configurations (fft1D 1024 Complex Double Forward then Inverse, four-step 2^22
+             row, Reverse/Inverse, fft2D 4096^2, batched
              fft2D 64 x 512^2, fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
              time and roofline fraction; the ragged batch and the spectral convolution also
              against the plain per-call path on the same GPU.
@@ -382,6 +383,13 @@ def main():
             add(f"fft_16384x4096_c64_{mode.name}", lambda m=mode: afft.fft(m, x), rows * n, alg_bytes, flops)
         del x, out
         torch.cuda.empty_cache()
+        # the reference's CPU-runnable case: fft1D Complex Double, N = 1024, Forward then Inverse
+        # (latency-bound: two launches of one workgroup each)
+        v = uniform_c((1024,), torch.complex128, 0, dev)
+        add("fft1D_1024_c128_forward_then_inverse",
+            lambda: afft.fft1D(Mode.Inverse, afft.fft1D(Mode.Forward, v)), 1024, 2 * 32.0 * 1024,
+            2 * 5.0 * 1024 * 10)
+        del v
         big = uniform_c((1, 1 << 22), torch.complex64, 2, dev)
         N = 1 << 22
         add("fft_1x4194304_c64_four_step", lambda: afft.fft(Mode.Forward, big), N, 16.0 * N, 5.0 * N * 22)
