@@ -53,7 +53,9 @@ def _device_tensor(arr):
     if arr.device.type != "cuda":
         raise ValueError("expected a tensor on a ROCm device")
     _dtype_code(arr)
-    return arr.contiguous()
+    # a conjugated or negated view keeps the original storage and only a flag: materialise it,
+    # or the library would read the unconjugated values behind the view
+    return arr.resolve_conj().resolve_neg().contiguous()
 
 
 def _stream(dev):

@@ -830,6 +830,10 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     auto take = [&](int ax) {
         return (axes == AFFT_AXES_ALL) || ax == rank - 1 || (axes == AFFT_AXES_LAST2 && ax >= rank - 2);
     };
+    // the kernels index one transform with 32-bit lengths: a transformed extent of 2^31 or more
+    // is refused here, before any length is narrowed
+    for (int ax = 0; ax < rank; ++ax)
+        if (take(ax) && extents[ax] >= ((int64_t)1 << 31)) return AFFT_ERROR_UNSUPPORTED_SIZE;
     int splits = 0;
     {
         int64_t inner = 1;
@@ -864,17 +868,15 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
         a.es = 1;
         b.add_stockham(1, plan.total, a, a, false, Slot::kIn, Slot::kOut);
     }
-    if (b.status == AFFT_SUCCESS && plan.work_elems > 0) {
-        plan.work = b.alloc(plan.work_elems * (int64_t)b.esz());
-    }
-    if (b.status == AFFT_SUCCESS && plan.aux2_elems > 0) {
-        plan.aux2 = b.alloc(plan.aux2_elems * (int64_t)b.esz());
-    }
-    if (b.status == AFFT_SUCCESS && plan.aux_elems > 0) {
-        plan.aux = b.alloc(plan.aux_elems * (int64_t)b.esz());
-    }
+    // the work and auxiliary arrays are not allocated here: each execution leases its own
     if (b.status == AFFT_SUCCESS && hipStreamSynchronize(s) != hipSuccess) b.status = AFFT_ERROR_HIP;
     return b.status;
+}
+
+static void free_block(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
 }
 
 void free_plan(Plan &plan)
@@ -883,20 +885,49 @@ void free_plan(Plan &plan)
     plan.allocs.clear();
     plan.launches.clear();
     plan.tw_cache.clear();
+    plan.pow_cache.clear();
     plan.blue_cache.clear();
+    if (plan.pool) plan.pool->clear(free_block);
 }
 
-static void *slot_ptr(Slot s, const void *in, void *out, const Plan &plan)
+static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+int64_t workspace_bytes(const Plan &plan)
+{
+    const int64_t esz = plan.dtype == AFFT_C64 ? 8 : 16;
+    return align256(plan.work_elems * esz) + align256(plan.aux_elems * esz) + align256(plan.aux2_elems * esz);
+}
+
+Workspace carve_workspace(const Plan &plan, void *base)
+{
+    const int64_t esz = plan.dtype == AFFT_C64 ? 8 : 16;
+    char *p = static_cast<char *>(base);
+    Workspace w;
+    if (!p) return w;
+    if (plan.work_elems) w.work = p;
+    p += align256(plan.work_elems * esz);
+    if (plan.aux_elems) w.aux = p;
+    p += align256(plan.aux_elems * esz);
+    if (plan.aux2_elems) w.aux2 = p;
+    return w;
+}
+
+static void *slot_ptr(Slot s, const void *in, void *out, const Workspace &ws)
 {
     switch (s) {
     case Slot::kIn: return const_cast<void *>(in);
     case Slot::kOut: return out;
-    case Slot::kWork: return plan.work;
-    case Slot::kAux: return plan.aux;
-    case Slot::kAux2: return plan.aux2;
+    case Slot::kWork: return ws.work;
+    case Slot::kAux: return ws.aux;
+    case Slot::kAux2: return ws.aux2;
     }
     return nullptr;
 }
+
+// the launches of a plan with a given workspace
+static int enqueue_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
+                        hipStream_t s, const int64_t *in_rows, const int64_t *out_rows,
+                        const SignSpec *sg_in, const SignSpec *sg_out, const Workspace &ws);
 
 static FastDiv make_fastdiv(uint64_t d)
 {
@@ -928,11 +959,40 @@ bool make_sign_spec(int rank, const int64_t *extents, SignSpec &out)
 
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
               hipStream_t s, const int64_t *in_rows, const int64_t *out_rows, const SignSpec *sg_in,
-              const SignSpec *sg_out)
+              const SignSpec *sg_out, const Workspace *ws)
 {
     if ((in_rows || out_rows) && (plan.rank != 2 || plan.axes != AFFT_AXES_INNERMOST))
         return AFFT_ERROR_INTERNAL;
     if (plan.total == 0) return AFFT_SUCCESS;
+    const int64_t wbytes = workspace_bytes(plan);
+    if (ws || wbytes == 0)
+        return enqueue_plan(plan, direction, d_in, d_out, scale, s, in_rows, out_rows, sg_in, sg_out,
+                            ws ? *ws : Workspace{});
+    // lease this execution's own workspace (one per stream in flight, see afft_pool.hpp)
+    auto make = [wbytes](void *&mem) {
+        mem = nullptr;
+        return hipMalloc(&mem, (size_t)wbytes) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_ALLOC;
+    };
+    WorkspacePool::Entry *e = nullptr;
+    const bool capturing = stream_capturing(s);
+    int st = capturing ? plan.pool->acquire_pinned(make, e) : plan.pool->acquire(s, make, e);
+    if (st) return st;
+    st = enqueue_plan(plan, direction, d_in, d_out, scale, s, in_rows, out_rows, sg_in, sg_out,
+                      carve_workspace(plan, e->res));
+    plan.pool->release(e, s, true);
+    return st;
+}
+
+static int enqueue_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
+                        hipStream_t s, const int64_t *in_rows, const int64_t *out_rows,
+                        const SignSpec *sg_in, const SignSpec *sg_out, const Workspace &ws)
+{
+    for (const Launch &l : plan.launches) {
+        const Slot used[2] = {l.src, l.dst};
+        for (Slot u : used)
+            if ((u == Slot::kWork && !ws.work) || (u == Slot::kAux && !ws.aux) || (u == Slot::kAux2 && !ws.aux2))
+                return AFFT_ERROR_INTERNAL;
+    }
     // in place (d_in == d_out) needs no private copy: every pass that reads the input either runs
     // in place, each workgroup reading all its elements before writing them, or writes the work
     // buffer, and the permuting passes read only the plan's own buffers
@@ -952,8 +1012,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
         hipError_t e;
         if (l.kind == Launch::kStockham || l.kind == Launch::kRegCol) {
             FftArgs a = l.fa;
-            a.in = slot_ptr(l.src, src_in, d_out, plan);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.in = slot_ptr(l.src, src_in, d_out, ws);
+            a.out = slot_ptr(l.dst, src_in, d_out, ws);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -967,8 +1027,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
                                           : launch_stockham(plan.dtype, a, l.block, l.grid, l.lds, s);
         } else if (l.kind == Launch::kRows) {
             RowArgs a = l.ra;
-            a.in = slot_ptr(l.src, src_in, d_out, plan);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.in = slot_ptr(l.src, src_in, d_out, ws);
+            a.out = slot_ptr(l.dst, src_in, d_out, ws);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -988,8 +1048,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             e = launch_rows(plan.dtype, l.row_n, l.row_variant, a, l.grid, s);
         } else if (l.kind == Launch::kTranspose) {
             TransposeArgs a = l.ta;
-            a.in = slot_ptr(l.src, src_in, d_out, plan);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.in = slot_ptr(l.src, src_in, d_out, ws);
+            a.out = slot_ptr(l.dst, src_in, d_out, ws);
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_out = last ? conj : 0;
             a.scale = last ? scale : 1.0;
@@ -999,8 +1059,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             e = launch_transpose(plan.dtype, a, s);
         } else if (l.kind == Launch::kChirp) {
             ChirpArgs a = l.ca;
-            a.in = slot_ptr(l.src, src_in, d_out, plan);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.in = slot_ptr(l.src, src_in, d_out, ws);
+            a.out = slot_ptr(l.dst, src_in, d_out, ws);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -1013,8 +1073,8 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
             e = launch_chirp(plan.dtype, a, s);
         } else {
             BluesteinArgs a = l.ba;
-            a.in = slot_ptr(l.src, src_in, d_out, plan);
-            a.out = slot_ptr(l.dst, src_in, d_out, plan);
+            a.in = slot_ptr(l.src, src_in, d_out, ws);
+            a.out = slot_ptr(l.dst, src_in, d_out, ws);
             a.nt_in = nt_enabled && l.src == Slot::kIn;
             a.nt_out = nt_enabled && last && l.dst == Slot::kOut;
             a.conj_in = first ? conj : 0;
@@ -1225,6 +1285,12 @@ static int check_shape(int dtype, int rank, const int64_t *extents, int max_rank
     if (!extents) return AFFT_ERROR_INVALID_VALUE;
     for (int i = 0; i < rank; ++i)
         if (extents[i] < 0) return AFFT_ERROR_INVALID_VALUE;
+    // the array's size in bytes must fit an int64_t (an empty array always does)
+    int64_t bytes = 16;
+    for (int i = 0; i < rank; ++i) {
+        if (extents[i] == 0) return AFFT_SUCCESS;
+        if (__builtin_mul_overflow(bytes, extents[i], &bytes)) return AFFT_ERROR_UNSUPPORTED_SIZE;
+    }
     return AFFT_SUCCESS;
 }
 
@@ -1235,8 +1301,10 @@ static int64_t total_of(int rank, const int64_t *e)
     return t;
 }
 
-// The front-end operation: mode semantics of FFT.hs (Inverse divides by `divisor`).
-static int front_end(int mode, int dtype, int rank, const int64_t *extents, int axes, int64_t divisor,
+// The front-end operation: mode semantics of FFT.hs.  Inverse divides by the product of the
+// extents from axis `div_from` inward: the innermost extent for fft (FFT.hs:69), every extent for
+// fft1D/fft2D/fft3D (FFT.hs:98,125,155), the two innermost for the batched fft2D.
+static int front_end(int mode, int dtype, int rank, const int64_t *extents, int axes, int div_from,
                      const void *in, void *out, void *stream)
 {
     if (!valid_mode(mode)) return AFFT_ERROR_INVALID_VALUE;
@@ -1249,6 +1317,7 @@ static int front_end(int mode, int dtype, int rank, const int64_t *extents, int 
     const Plan *p = nullptr;
     if ((st = cached_plan(p, dtype, rank, extents, axes, dev))) return st;
     const int dir = mode == AFFT_FORWARD ? -1 : 1;
+    const int64_t divisor = total_of(rank - div_from, extents + div_from);
     const double scale = (mode == AFFT_INVERSE && divisor > 0) ? 1.0 / (double)divisor : 1.0;
     return run_plan(*p, dir, in, out, scale, stream);
 }
@@ -1258,34 +1327,34 @@ extern "C" int afft_fft(int mode, int dtype, int rank, const int64_t *extents, c
 {
     int st = check_shape(dtype, rank, extents, 8);
     if (st) return st;
-    return front_end(mode, dtype, rank, extents, AFFT_AXES_INNERMOST, extents[rank - 1], d_in, d_out, stream);
+    return front_end(mode, dtype, rank, extents, AFFT_AXES_INNERMOST, rank - 1, d_in, d_out, stream);
 }
 
 extern "C" int afft_fft1D(int mode, int dtype, int64_t n, const void *d_in, void *d_out, void *stream)
 {
     const int64_t e[1] = {n};
-    return front_end(mode, dtype, 1, e, AFFT_AXES_ALL, n, d_in, d_out, stream);
+    return front_end(mode, dtype, 1, e, AFFT_AXES_ALL, 0, d_in, d_out, stream);
 }
 
 extern "C" int afft_fft2D(int mode, int dtype, int64_t h, int64_t w, const void *d_in, void *d_out,
                           void *stream)
 {
     const int64_t e[2] = {h, w};
-    return front_end(mode, dtype, 2, e, AFFT_AXES_ALL, h * w, d_in, d_out, stream);
+    return front_end(mode, dtype, 2, e, AFFT_AXES_ALL, 0, d_in, d_out, stream);
 }
 
 extern "C" int afft_fft3D(int mode, int dtype, int64_t d, int64_t h, int64_t w, const void *d_in,
                           void *d_out, void *stream)
 {
     const int64_t e[3] = {d, h, w};
-    return front_end(mode, dtype, 3, e, AFFT_AXES_ALL, d * h * w, d_in, d_out, stream);
+    return front_end(mode, dtype, 3, e, AFFT_AXES_ALL, 0, d_in, d_out, stream);
 }
 
 extern "C" int afft_fft2D_many(int mode, int dtype, int64_t batch, int64_t h, int64_t w,
                                const void *d_in, void *d_out, void *stream)
 {
     const int64_t e[3] = {batch, h, w};
-    return front_end(mode, dtype, 3, e, AFFT_AXES_LAST2, h * w, d_in, d_out, stream);
+    return front_end(mode, dtype, 3, e, AFFT_AXES_LAST2, 1, d_in, d_out, stream);
 }
 
 extern "C" int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
@@ -1354,7 +1423,7 @@ extern "C" int afft_plan_info(const afft_plan *plan, int *launches, int64_t *dev
     if (!plan) return AFFT_ERROR_INVALID_VALUE;
     const Plan *p = reinterpret_cast<const Plan *>(plan);
     if (launches) *launches = (int)p->launches.size();
-    if (device_bytes) *device_bytes = p->device_bytes;
+    if (device_bytes) *device_bytes = p->device_bytes + workspace_bytes(*p);
     return AFFT_SUCCESS;
 }
 

@@ -5,10 +5,12 @@
 #include <stdint.h>
 
 #include <map>
+#include <memory>
 #include <utility>
 #include <vector>
 
 #include "afft_internal.hpp"
+#include "afft_pool.hpp"
 
 namespace afft {
 
@@ -41,6 +43,17 @@ struct Launch {
     size_t lds;
 };
 
+// The device arrays one execution of a plan writes besides its output: the work array of split
+// axes and the two auxiliary arrays of multi-kernel Bluestein transforms.  They belong to the
+// execution, not to the plan: each execution leases its own (afft_pool.hpp), or the caller brings
+// them (ragged batches, the allocation-free entry point).
+struct Workspace {
+    void *work = nullptr;
+    void *aux = nullptr;
+    void *aux2 = nullptr;
+};
+using WorkspacePool = ExecPool<void *>; // a lease is one device block, carved into a Workspace
+
 // Plan::flags: the plan runs on gathered rows that start at arbitrary offsets (ragged batches),
 // so short row segments of a transposed store would straddle cache lines
 constexpr int kPlanUnalignedRows = 1;
@@ -56,13 +69,17 @@ struct Plan {
     std::map<int64_t, const void *> tw_cache;
     std::map<int64_t, const void *> pow_cache;
     std::map<int64_t, std::pair<const void *, const void *>> blue_cache;
+    // elements of each per-execution array (0: the plan needs none of it)
     int64_t work_elems = 0;
-    void *work = nullptr;
     int64_t aux_elems = 0;
-    void *aux = nullptr;
     int64_t aux2_elems = 0;  // partner of aux for four-step splits inside a Bluestein transform
-    void *aux2 = nullptr;
+    std::shared_ptr<WorkspacePool> pool = std::make_shared<WorkspacePool>();
 };
+
+// Bytes of one execution's workspace (the three arrays, each 256-byte aligned), 0 if none.
+int64_t workspace_bytes(const Plan &plan);
+// The three arrays laid out in a block of workspace_bytes(plan) at base.
+Workspace carve_workspace(const Plan &plan, void *base);
 
 // Single-workgroup layout of length n: fewest passes, then E closest to 16; optional limits on
 // elements per thread and threads per transform.
@@ -77,9 +94,11 @@ void free_plan(Plan &plan);
 // offsets of the count rows in d_in / d_out, for the rows of one length of a ragged batch.
 // sg_in / sg_out (optional): a centring sign applied on the first pass's loads / the last pass's
 // stores (fused DFT/Centre.hs operations).
+// ws (optional): the workspace to use; without it the execution leases one from the plan's pool.
 int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, double scale,
               hipStream_t s, const int64_t *in_rows = nullptr, const int64_t *out_rows = nullptr,
-              const SignSpec *sg_in = nullptr, const SignSpec *sg_out = nullptr);
+              const SignSpec *sg_in = nullptr, const SignSpec *sg_out = nullptr,
+              const Workspace *ws = nullptr);
 // the sign (-1)^(sum of the multi-index) over a row-major array of rank <= 3 (false: too large)
 bool make_sign_spec(int rank, const int64_t *extents, SignSpec &out);
 int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);

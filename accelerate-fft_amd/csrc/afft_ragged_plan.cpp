@@ -40,6 +40,29 @@ struct RaggedLong {
 // took it from 550 to 465 us).
 constexpr int64_t kSmallLongElems = 1 << 23;
 
+// The per-execution resources of a ragged batch: a side stream forked from and joined back to
+// the caller's stream, with its two events, and one device block holding the workspace of every
+// long class.  Each execution leases a set (afft_pool.hpp), so two executions of one plan on two
+// streams never share a side stream or a workspace.
+struct RaggedExec {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    void *mem = nullptr;
+};
+using RaggedPool = ExecPool<RaggedExec>;
+
+// At most this many captured graphs per plan (keyed by mode and buffers); the least recently
+// replayed one is dropped, after its last replay has finished, to make room for a new key.
+constexpr size_t kMaxGraphs = 8;
+
+struct RaggedGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    RaggedPool::Entry *set = nullptr; // the resources captured into the graph
+    hipEvent_t launched = nullptr;    // recorded after the last replay
+    uint64_t last_use = 0;
+};
+
 struct RaggedPlan {
     int dtype = 0, device = 0;
     int64_t count = 0, total = 0;
@@ -51,14 +74,12 @@ struct RaggedPlan {
     int64_t grid = 0;
     size_t lds = 0;
     std::vector<std::unique_ptr<RaggedLong>> longs;
-    // Latency-bound work (the persistent launch, long classes too small to fill the device) runs
-    // on a side stream forked from and joined back to the caller's stream, beside the
-    // bandwidth-bound long classes; launch_mu makes each fork/join pair atomic across threads.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    mutable std::mutex launch_mu;
+    std::vector<int64_t> ws_off;  // offset of each long class's workspace in RaggedExec::mem
+    int64_t ws_bytes = 0;
+    RaggedPool pool;
     std::mutex graph_mu;
-    std::map<std::tuple<int, const void *, void *>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    std::map<std::tuple<int, const void *, void *>, RaggedGraph> graphs;
+    uint64_t graph_clock = 0;
 };
 
 namespace {
@@ -206,48 +227,77 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
     if (!st) rp.d_offsets = upload(rp, dev, moffs, s, st);
     if (st) return st;
     for (size_t i = 0; i < rp.longs.size(); ++i) rp.longs[i]->rows = rp.d_offsets + long_first[i];
+    for (const auto &lp : rp.longs) {
+        rp.ws_off.push_back(rp.ws_bytes);
+        rp.ws_bytes += workspace_bytes(lp->plan);
+    }
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
     rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
-    if (hipStreamCreateWithFlags(&rp.side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&rp.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&rp.join, hipEventDisableTiming) != hipSuccess)
-        return AFFT_ERROR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
 
+static int make_exec(const RaggedPlan &rp, RaggedExec &x)
+{
+    x = RaggedExec{};
+    if (hipStreamCreateWithFlags(&x.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&x.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&x.join, hipEventDisableTiming) != hipSuccess)
+        return AFFT_ERROR_HIP;
+    if (rp.ws_bytes > 0 && hipMalloc(&x.mem, (size_t)rp.ws_bytes) != hipSuccess) return AFFT_ERROR_ALLOC;
+    return AFFT_SUCCESS;
+}
+
+static void free_exec(RaggedExec &x)
+{
+    if (x.side) {
+        (void)hipStreamSynchronize(x.side);
+        (void)hipStreamDestroy(x.side);
+    }
+    if (x.fork) (void)hipEventDestroy(x.fork);
+    if (x.join) (void)hipEventDestroy(x.join);
+    if (x.mem) (void)hipFree(x.mem);
+    x = RaggedExec{};
+}
+
 static void destroy_ragged(RaggedPlan *rp)
 {
-    if (rp->side) (void)hipStreamSynchronize(rp->side);
+    for (auto &kv : rp->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        if (kv.second.launched) (void)hipEventDestroy(kv.second.launched);
+    }
+    rp->graphs.clear();
+    rp->pool.clear(free_exec);
     for (auto &lp : rp->longs) free_plan(lp->plan);
     free_plan(rp->tables);
-    if (rp->fork) (void)hipEventDestroy(rp->fork);
-    if (rp->join) (void)hipEventDestroy(rp->join);
-    if (rp->side) (void)hipStreamDestroy(rp->side);
     delete rp;
 }
 
-static int run_long(const RaggedLong &lp, int mode, const void *in, void *out, hipStream_t s)
+static int run_long(const RaggedLong &lp, int mode, const void *in, void *out, hipStream_t s, const Workspace &ws)
 {
     const int direction = mode == AFFT_FORWARD ? -1 : +1;
     const double scale = mode == AFFT_INVERSE ? 1.0 / (double)lp.L : 1.0;
-    return exec_plan(lp.plan, direction, in, out, scale, s, lp.rows, lp.rows);
+    return exec_plan(lp.plan, direction, in, out, scale, s, lp.rows, lp.rows, nullptr, nullptr, &ws);
 }
 
-// The persistent launch and the small long classes on the side stream, the other long classes on
-// s, joined back into s.  Every transform reads and writes only its own rows, so the branches
-// touch disjoint memory and in-place execution (in == out) needs no copy.
-static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s)
+// The persistent launch and the small long classes on the set's side stream, the other long
+// classes on s, joined back into s.  Every transform reads and writes only its own rows, so the
+// branches touch disjoint memory and in-place execution (in == out) needs no copy.
+static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s,
+                         const RaggedExec &x)
 {
-    std::lock_guard<std::mutex> lk(rp.launch_mu);
-    hipStream_t side = rp.side;
-    if (hipEventRecord(rp.fork, s) != hipSuccess || hipStreamWaitEvent(side, rp.fork, 0) != hipSuccess)
+    hipStream_t side = x.side;
+    if (hipEventRecord(x.fork, s) != hipSuccess || hipStreamWaitEvent(side, x.fork, 0) != hipSuccess)
         return AFFT_ERROR_HIP;
+    auto ws_of = [&](size_t i) {
+        return carve_workspace(rp.longs[i]->plan, x.mem ? static_cast<char *>(x.mem) + rp.ws_off[i] : nullptr);
+    };
     int st = AFFT_SUCCESS;
     // the bandwidth-bound long classes on the caller's stream first: they are the critical path,
     // and enqueued first they start without waiting for the side stream's launches to be issued
-    for (const auto &lp : rp.longs)
-        if (!st && !lp->small) st = run_long(*lp, mode, in, out, s);
+    for (size_t i = 0; i < rp.longs.size(); ++i)
+        if (!st && !rp.longs[i]->small) st = run_long(*rp.longs[i], mode, in, out, s, ws_of(i));
     if (!st && rp.nitems > 0) {
         RaggedArgs a{};
         a.in = in;
@@ -260,10 +310,10 @@ static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *o
         a.inverse = mode == AFFT_INVERSE ? 1 : 0;
         if (launch_ragged(rp.dtype, a, rp.grid, rp.lds, side) != hipSuccess) st = AFFT_ERROR_HIP;
     }
-    for (const auto &lp : rp.longs)
-        if (!st && lp->small) st = run_long(*lp, mode, in, out, side);
+    for (size_t i = 0; i < rp.longs.size(); ++i)
+        if (!st && rp.longs[i]->small) st = run_long(*rp.longs[i], mode, in, out, side, ws_of(i));
     // join even after an error, so that the side stream never runs ahead of the caller's
-    if (hipEventRecord(rp.join, side) != hipSuccess || hipStreamWaitEvent(s, rp.join, 0) != hipSuccess)
+    if (hipEventRecord(x.join, side) != hipSuccess || hipStreamWaitEvent(s, x.join, 0) != hipSuccess)
         st = st ? st : AFFT_ERROR_HIP;
     return st;
 }
@@ -300,11 +350,9 @@ extern "C" int afft_ragged_plan_create(afft_ragged_plan **plan, int dtype, int64
     st = build_ragged(*rp, lengths, dev, s);
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
+    if (st) destroy_ragged(rp);
     if (old >= 0) (void)hipSetDevice(old);
-    if (st) {
-        destroy_ragged(rp);
-        return st;
-    }
+    if (st) return st;
     *plan = reinterpret_cast<afft_ragged_plan *>(rp);
     return AFFT_SUCCESS;
 }
@@ -314,21 +362,29 @@ extern "C" int afft_ragged_exec(const afft_ragged_plan *plan, int mode, const vo
 {
     if (!plan) return AFFT_ERROR_INVALID_VALUE;
     if (mode != AFFT_FORWARD && mode != AFFT_REVERSE && mode != AFFT_INVERSE) return AFFT_ERROR_INVALID_VALUE;
-    const RaggedPlan *rp = reinterpret_cast<const RaggedPlan *>(plan);
+    RaggedPlan *rp = const_cast<RaggedPlan *>(reinterpret_cast<const RaggedPlan *>(plan));
     if (rp->total == 0) return AFFT_SUCCESS;
     if (!d_in || !d_out) return AFFT_ERROR_INVALID_VALUE;
     int old = -1;
     (void)hipGetDevice(&old);
     if (old != rp->device && hipSetDevice(rp->device) != hipSuccess) return AFFT_ERROR_HIP;
-    const int st = ragged_launch(*rp, mode, d_in, d_out, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    auto make = [rp](RaggedExec &x) { return make_exec(*rp, x); };
+    RaggedPool::Entry *e = nullptr;
+    int st = stream_capturing(s) ? rp->pool.acquire_pinned(make, e) : rp->pool.acquire(s, make, e);
+    if (!st) {
+        st = ragged_launch(*rp, mode, d_in, d_out, s, e->res);
+        rp->pool.release(e, s, true);
+    }
     if (old >= 0 && old != rp->device) (void)hipSetDevice(old);
     return st;
 }
 
-// Capture the launches once per (mode, in, out) on a private stream and replay the instantiated
-// graph on the caller's stream (capture is not allowed on the legacy default stream).  The graph
-// holds only kernel nodes: no kernel keeps device-side state, so there is no memset node to
-// replay.
+// Capture the launches once per (mode, in, out) on a private stream, with a resource set of the
+// graph's own, and replay the instantiated graph on the caller's stream (capture is not allowed on
+// the legacy default stream).  The graph holds only kernel nodes and the fork/join of its own side
+// stream: no kernel keeps device-side state, so there is no memset node to replay.  Replays of one
+// graph are ordered through its `launched` event, since they share the graph's workspace.
 extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const void *d_in, void *d_out,
                                       void *stream)
 {
@@ -340,30 +396,54 @@ extern "C" int afft_ragged_exec_graph(afft_ragged_plan *plan, int mode, const vo
     int old = -1;
     (void)hipGetDevice(&old);
     if (old != rp->device && hipSetDevice(rp->device) != hipSuccess) return AFFT_ERROR_HIP;
+    hipStream_t s = (hipStream_t)stream;
     int st = AFFT_SUCCESS;
-    hipGraphExec_t exec = nullptr;
     {
         std::lock_guard<std::mutex> lk(rp->graph_mu);
         auto key = std::make_tuple(mode, d_in, d_out);
         auto it = rp->graphs.find(key);
-        if (it != rp->graphs.end()) {
-            exec = it->second.second;
-        } else {
-            hipStream_t cap;
-            if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) st = AFFT_ERROR_HIP;
-            hipGraph_t g = nullptr;
+        if (it == rp->graphs.end()) {
+            if (rp->graphs.size() >= kMaxGraphs) {
+                auto lru = rp->graphs.begin();
+                for (auto j = rp->graphs.begin(); j != rp->graphs.end(); ++j)
+                    if (j->second.last_use < lru->second.last_use) lru = j;
+                RaggedGraph &g = lru->second;
+                (void)hipEventSynchronize(g.launched);
+                (void)hipGraphExecDestroy(g.exec);
+                (void)hipGraphDestroy(g.graph);
+                (void)hipEventDestroy(g.launched);
+                rp->pool.drop(g.set, free_exec);
+                rp->graphs.erase(lru);
+            }
+            RaggedGraph g;
+            auto make = [rp](RaggedExec &x) { return make_exec(*rp, x); };
+            st = rp->pool.acquire_pinned(make, g.set);
+            hipStream_t cap = nullptr;
+            if (!st && hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) st = AFFT_ERROR_HIP;
             if (!st && hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) st = AFFT_ERROR_HIP;
             if (!st) {
-                const int lst = ragged_launch(*rp, mode, d_in, d_out, cap);
-                if (hipStreamEndCapture(cap, &g) != hipSuccess || lst) st = lst ? lst : AFFT_ERROR_HIP;
+                const int lst = ragged_launch(*rp, mode, d_in, d_out, cap, g.set->res);
+                if (hipStreamEndCapture(cap, &g.graph) != hipSuccess || lst) st = lst ? lst : AFFT_ERROR_HIP;
             }
-            if (!st && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) st = AFFT_ERROR_HIP;
-            if (!st) rp->graphs[key] = {g, exec};
-            else if (g) (void)hipGraphDestroy(g);
-            (void)hipStreamDestroy(cap);
+            if (!st && hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) st = AFFT_ERROR_HIP;
+            if (!st && hipEventCreateWithFlags(&g.launched, hipEventDisableTiming) != hipSuccess) st = AFFT_ERROR_HIP;
+            if (cap) (void)hipStreamDestroy(cap);
+            if (st) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                if (g.graph) (void)hipGraphDestroy(g.graph);
+                if (g.set) rp->pool.drop(g.set, free_exec);
+            } else {
+                it = rp->graphs.emplace(key, g).first;
+            }
+        }
+        if (!st) {
+            RaggedGraph &g = it->second;
+            g.last_use = ++rp->graph_clock;
+            if (hipStreamWaitEvent(s, g.launched, 0) != hipSuccess || hipGraphLaunch(g.exec, s) != hipSuccess ||
+                hipEventRecord(g.launched, s) != hipSuccess)
+                st = AFFT_ERROR_HIP;
         }
     }
-    if (!st && hipGraphLaunch(exec, (hipStream_t)stream) != hipSuccess) st = AFFT_ERROR_HIP;
     if (old >= 0 && old != rp->device) (void)hipSetDevice(old);
     return st;
 }
@@ -376,10 +456,6 @@ extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *plan)
     (void)hipGetDevice(&old);
     (void)hipSetDevice(rp->device);
     (void)hipDeviceSynchronize();
-    for (auto &kv : rp->graphs) {
-        (void)hipGraphExecDestroy(kv.second.second);
-        (void)hipGraphDestroy(kv.second.first);
-    }
     destroy_ragged(rp);
     if (old >= 0) (void)hipSetDevice(old);
     return AFFT_SUCCESS;

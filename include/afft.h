@@ -141,8 +141,13 @@ int afft_dftGS(int dtype, int rank, const int64_t *extents, const int64_t *index
 
 /* Build an immutable plan: kernels chosen, twiddles generated on the device in FP64 and rounded
  * to the element type.  `axes` is AFFT_AXES_INNERMOST, AFFT_AXES_ALL or AFFT_AXES_LAST2 (rank
- * >= 2).  `device` is a HIP device ordinal.  Plans carry no stream state, so one
- * plan may be executed concurrently on several streams.
+ * >= 2).  `device` is a HIP device ordinal.
+ * Concurrency: a plan carries no stream state, and the intermediate arrays of an execution (the
+ * work array of a split axis, the auxiliary arrays of a multi-kernel Bluestein transform) are
+ * leased per execution from a small per-plan pool, one set per stream in flight (an execution
+ * on another stream waits for a set only when four are in flight at once).  So one plan -- a
+ * cached one included -- may be executed from several host threads on several streams at once.
+ * An execution captured into a graph keeps a set of its own for as long as the plan lives.
  * Replaces the cuFFT handle creation of LLVM/PTX.hs:137-170 and PTX/Plans.hs:40-90. */
 int afft_plan_create(afft_plan **plan, int dtype, int rank, const int64_t *extents, int axes,
                      int device);
@@ -164,8 +169,9 @@ int afft_plan_destroy(afft_plan *plan);
  * cached plan. */
 int afft_cache_clear(void);
 
-/* Number of kernel launches one afft_exec of the plan makes and the bytes of device workspace it
- * holds (twiddles, Bluestein filters, intermediate buffers). */
+/* Number of kernel launches one afft_exec of the plan makes, and the bytes of device memory it
+ * needs: its tables (twiddles, Bluestein filters, row tables) plus the intermediate arrays of one
+ * execution (each stream executing the plan concurrently leases one such set). */
 int afft_plan_info(const afft_plan *plan, int *launches, int64_t *device_bytes);
 
 /* ---- host-only planning (no device needed) ---------------------------------------------- */
