@@ -54,6 +54,7 @@ def lib():
     _sig(L.afft_plan_create, i, pvp, i, i, pi64, i, i)
     _sig(L.afft_plan_cached, i, pvp, i, i, pi64, i, i)
     _sig(L.afft_exec, i, vp, i, vp, vp, d, vp)
+    _sig(L.afft_exec_noalloc, i, vp, i, vp, vp, d, vp)
     _sig(L.afft_plan_destroy, i, vp)
     _sig(L.afft_cache_clear, i)
     _sig(L.afft_plan_info, i, vp, ctypes.POINTER(ctypes.c_int), pi64)

@@ -1272,6 +1272,7 @@ extern "C" const char *afft_strerror(int status)
     case AFFT_ERROR_NO_DEVICE: return "no usable device";
     case AFFT_ERROR_IO: return "plan-cache file error";
     case AFFT_ERROR_INTERNAL: return "internal error";
+    case AFFT_ERROR_SCRATCH_TOO_SMALL: return "the plan's intermediate arrays exceed the allocation-free scratch";
     default: return "unknown status";
     }
 }

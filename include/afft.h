@@ -42,6 +42,7 @@ extern "C" {
 #define AFFT_ERROR_NO_DEVICE 7        /* no usable gfx950 device                              */
 #define AFFT_ERROR_IO 8               /* plan-cache file could not be read or written        */
 #define AFFT_ERROR_INTERNAL 9
+#define AFFT_ERROR_SCRATCH_TOO_SMALL 10 /* afft_exec_noalloc: the plan needs more than the scratch */
 
 /* ---- enums ------------------------------------------------------------------------------ */
 /* element type: NumericRfloat32 / NumericRfloat64 (FFT/Type.hs:28-30) */
@@ -164,6 +165,19 @@ int afft_exec(const afft_plan *plan, int direction, const void *d_in, void *d_ou
               void *stream);
 
 int afft_plan_destroy(afft_plan *plan);
+
+/* afft_exec without any device allocation: the intermediate arrays come from a fixed scratch array
+ * of AFFT_NOALLOC_SCRATCH_BYTES per GPU (a static device array of the library, held until the
+ * process ends) instead of the plan's per-stream pool.  Plans whose intermediates exceed it (see
+ * afft_plan_info: device bytes beyond the tables) return AFFT_ERROR_SCRATCH_TOO_SMALL before any
+ * launch.  Every call on one GPU shares that GPU's scratch, so calls that overlap on a GPU (two
+ * streams or two threads at once) race on it: the caller serialises them.  The plan itself is
+ * created beforehand (afft_plan_create / afft_plan_cached allocate its tables).
+ * No counterpart in the reference: for callers that cannot allocate device memory during
+ * execution (code under graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define AFFT_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int afft_exec_noalloc(const afft_plan *plan, int direction, const void *d_in, void *d_out,
+                      double scale, void *stream);
 
 /* Drop every cached plan (and its device memory).  Not safe while another thread is executing a
  * cached plan. */
