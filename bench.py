@@ -4,8 +4,9 @@
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu] [--no-extras]
 
 Workload (the headline line): `fft` Forward along the innermost axis of an Array DIM2
-(Complex Float) of 16384 x 4096, re/im uniform[-1, 1) -- 16384 independent 4096-point
-transforms, inputs resident in HBM before the timed region.  One step = one call.
+(Complex Float) of 16384 x 4096, re/im uniform[-1, 1) from the counter-based SplitMix64 stream of
+seed 2 -- 16384 independent 4096-point transforms, inputs resident in HBM before the timed region.
+One step = one call.
 
   value      GFLOP/s by the 5 N log2 N convention: 16384 * 5 * 4096 * 12 flop per step.
   roofline   HBM-bound; achieved = algorithmic bytes (one read + one write of the array,
@@ -13,21 +14,26 @@ transforms, inputs resident in HBM before the timed region.  One step = one call
              HIP events on the stream the kernel is launched on; peak 8 TB/s.  `traffic` is the
              HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a
              copy kernel of known size), read from profiles/ when present.
-  cpu_baseline  the oracle's recursive FFT in float arithmetic (the CPU path restated; FFTW is
-             not installed), over the same 16384 x 4096 array, on every host core available.
-  extras     the other configurations (fft1D 1024 Complex Double Forward then Inverse, four-step 2^22
-             row, Reverse/Inverse, fft2D 4096^2, batched
-             fft2D 64 x 512^2, fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their own
-             time and roofline fraction; the ragged batch and the spectral convolution also
-             against the plain per-call path on the same GPU.
+  rel_l2     every result's relative L2 error against the FP64 oracle on the same (rounded)
+             inputs, computed after the timed region, with the bound it must meet (1e-6 log2 N for
+             Complex Float, 1e-13 for Complex Double).
+  cpu_baseline  the oracle's recursive FFT in float arithmetic (the CPU path restated, `kind`
+             "port") over a sample of the same array, on every host core and on one; beside it,
+             scipy.fft (pocketfft), an FFTW-class library standing in for FFTW (absent here),
+             labelled a substitute, with its plan time apart.
+  extras     the other configurations (fft1D 1024 Complex Double Forward then Inverse, the
+             four-step 2^22 row in three modes, Reverse/Inverse, fft2D 4096^2, batched fft2D
+             64 x 512^2, fft3D 256^3 Complex Double, ragged batch, spectral convolution) with their
+             own time, roofline fraction and error; the ragged batch and the spectral convolution
+             also against the plain per-call path on the same GPU.
 
   Before the W warmup steps, untimed steps run for 50 ms so that a fresh process's clocks have
   ramped up (the first ~30 calls otherwise run 5-8% slower).
 
   --profile-headline runs only the headline plan (and a plain device copy of the same array,
              the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
-  --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral) for
-             warmup + steps calls, for profiling.
+  --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral,
+             batched2d) for warmup + steps calls, for profiling.
 """
 import argparse
 import json
@@ -41,6 +47,7 @@ sys.path.insert(0, os.path.join(ROOT, "accelerate-fft_amd"))
 
 HBM_PEAK_GBS = 8000.0
 RAGGED_SIZES = sorted({b << k for b in (1, 3, 5) for k in range(4, 14)} | {257, 1009, 4099})
+METRIC = "FFT GFLOP/s (5N log2N) + % HBM roofline; fwd+adjoint time vs plain, same GPU"
 
 
 def ragged_lengths(count=10000, seed=4):
@@ -51,34 +58,139 @@ def ragged_lengths(count=10000, seed=4):
     u = rng.uniform(math.log(16), math.log(40960), count)
     logs = np.log(np.array(RAGGED_SIZES, dtype=np.float64))
     return [RAGGED_SIZES[int(np.argmin(np.abs(logs - v)))] for v in u]
-METRIC = "FFT GFLOP/s (5N log2N) + % HBM roofline; fwd+adjoint time vs plain, same GPU"
 
 
-def uniform_c(shape, dtype, seed, device):
+# ----------------------------------------------------------------------------------------------
+# Seeded inputs: the counter-based SplitMix64 stream (value i of stream `seed` is
+# mix(mix(seed) + i)); re of element i from counter 2i, im from 2i+1.  Uniform maps the top 53
+# bits to [0, 1) and then to [-1, 1), exactly; normal is Box-Muller over counter pairs.
+
+_CHUNK = 1 << 22
+
+
+def _mix64(z):
+    import numpy as np
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _stream(seed, count):
+    """The first `count` 64-bit values of stream `seed`, in chunks (bounded temporaries)."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        base = _mix64(np.uint64(seed))
+        for lo in range(0, count, _CHUNK):
+            hi = min(count, lo + _CHUNK)
+            yield lo, _mix64(base + np.arange(lo, hi, dtype=np.uint64))
+
+
+def splitmix_uniform(seed, shape, dtype):
+    import numpy as np
+    n = int(np.prod(shape, dtype=np.int64))
+    out = np.empty(2 * n, dtype=np.float64)
+    for lo, h in _stream(seed, 2 * n):
+        out[lo:lo + h.size] = 2.0 * ((h >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
+    return out.view(np.complex128).reshape(shape).astype(dtype)
+
+
+def splitmix_normal(seed, shape, dtype):
+    import numpy as np
+    n = int(np.prod(shape, dtype=np.int64))
+    out = np.empty(2 * n, dtype=np.float64)
+    for lo, h in _stream(seed, 2 * n):  # chunks are even, so pairs never straddle one
+        u1 = ((h[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0 ** -53
+        u2 = (h[1::2] >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        r = np.sqrt(-2.0 * np.log(u1))
+        out[lo:lo + h.size:2] = r * np.cos(2.0 * np.pi * u2)
+        out[lo + 1:lo + h.size:2] = r * np.sin(2.0 * np.pi * u2)
+    return out.view(np.complex128).reshape(shape).astype(dtype)
+
+
+def to_device(x, dev):
     import torch
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    real = torch.float32 if dtype == torch.complex64 else torch.float64
-    re = torch.rand(shape, generator=g, device=device, dtype=real) * 2 - 1
-    im = torch.rand(shape, generator=g, device=device, dtype=real) * 2 - 1
-    return torch.complex(re, im)
+    return torch.from_numpy(x).to(dev)
 
 
-def normal_c(shape, dtype, seed, device):
-    import torch
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    real = torch.float32 if dtype == torch.complex64 else torch.float64
-    return torch.complex(torch.randn(shape, generator=g, device=device, dtype=real),
-                         torch.randn(shape, generator=g, device=device, dtype=real))
+# ----------------------------------------------------------------------------------------------
+# The accuracy check: the FP64 oracle (test infrastructure) recomputes each result from the same
+# inputs after the timed region; it is the checker, never the thing timed.
+
+class Checker:
+    def __init__(self):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        self.O = oracle
+
+    @staticmethod
+    def tol(f64, n):
+        return 1e-13 if f64 else 1e-6 * max(1.0, math.log2(max(n, 2)))
+
+    def entry(self, got, want, f64, n):
+        err = self.O.rel_l2(got, want)
+        tol = self.tol(f64, n)
+        return {"rel_l2": float(f"{err:.3e}"), "tol": float(f"{tol:.3e}"), "ok": bool(err <= tol)}
+
+    def mode(self, m):
+        return {"Forward": self.O.Mode.Forward, "Reverse": self.O.Mode.Reverse,
+                "Inverse": self.O.Mode.Inverse}[m.name]
+
+    def ragged(self, got, x, lengths, m):
+        """Relative L2 over the whole packed batch, and the worst single transform as a multiple
+        of its own bound."""
+        import numpy as np
+        O = self.O
+        offs = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+        want = np.empty(x.shape, dtype=np.complex128)
+        by = {}
+        for i, n in enumerate(lengths):
+            by.setdefault(n, []).append(i)
+        worst = 0.0
+        for n, idx in by.items():
+            rows = (offs[idx][:, None] + np.arange(n)[None, :])
+            w = O.fft(self.mode(m), x[rows])
+            want[rows] = w
+            num = np.linalg.norm(got[rows] - w, axis=1)
+            den = np.maximum(np.linalg.norm(w, axis=1), 1e-300)
+            worst = max(worst, float(np.max(num / den)) / self.tol(False, n))
+        e = self.entry(got, want, False, max(lengths))
+        e["worst_row_over_tol"] = round(worst, 4)
+        e["ok"] = bool(worst <= 1.0)
+        return e
+
+    def spectral(self, x, H, t, y, gx, gH):
+        import numpy as np
+        O = self.O
+        M = H.size
+        FH = H
+        Fx = np.stack([O.fft2D(O.Mode.Forward, xb) for xb in x])
+        wy = np.stack([O.fft2D(O.Mode.Inverse, f * FH) for f in Fx])
+        g = 2.0 * (y.astype(np.complex128) - t)  # the cotangent the GPU step formed, in FP64
+        Fg = np.stack([O.fft2D(O.Mode.Forward, gb) for gb in g])
+        wgx = np.stack([O.fft2D(O.Mode.Reverse, np.conj(FH) * f) / M for f in Fg])
+        wgH = np.sum(np.conj(Fx) * Fg, axis=0) / M
+        n = x.shape[1] * x.shape[2]
+        return {"y": self.entry(y, wy, False, n), "gx": self.entry(gx, wgx, False, n),
+                "gH": self.entry(gH, wgH, False, n)}
 
 
-def time_calls(fn, steps, warmup):
+# ----------------------------------------------------------------------------------------------
+# Timing
+
+def time_calls(fn, steps, warmup, settle_s=0.05):
     """Average seconds per call, HIP events on the current stream (the one the library
-    launches on) bracketing `steps` calls after `warmup` untimed ones."""
+    launches on) bracketing `steps` calls after `warmup` untimed ones, and after at least
+    `settle_s` of untimed calls: the host-side accuracy checks between workloads leave the GPU
+    idle for seconds, and its clocks need that long to ramp up again."""
     import torch
-    for _ in range(warmup):
+    end = time.perf_counter() + settle_s
+    done = 0
+    while done < warmup or time.perf_counter() < end:
         fn()
+        done += 1
+        if done % 8 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     a = torch.cuda.Event(enable_timing=True)
@@ -105,27 +217,66 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(rows, n, min_seconds=12.0, max_reps=400):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def cpu_baseline(x, rows, n, budget_s=6.0):
+    """The CPU path timed on this host on a bounded sample of the headline array: the oracle's
+    float recursion (the port), on every core and on one, and scipy.fft (pocketfft) as an
+    FFTW-class substitute, on every core and on one, with its plan time apart."""
     import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    x = O.uniform(2, (rows, n), np.complex64)
-    O.fft_rows_f32(x[:64], -1, nthreads=cores)  # warm-up, twiddles
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        O.fft_rows_f32(x, -1, nthreads=cores)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds or reps >= max_reps:
-            break
-    flops = 5.0 * n * math.log2(n) * rows * reps
-    return {"value": round(flops / el / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+    flops_row = 5.0 * n * math.log2(n)
+
+    def rate(call, nrows):
+        call()  # warm-up (twiddles, plans)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            call()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or reps >= 200:
+                break
+        return flops_row * nrows * reps / el / 1e9, reps, el
+
+    full = x  # 16384 rows
+    few = x[:1024]
+    port_all, r1, e1 = rate(lambda: O.fft_rows_f32(full, -1, nthreads=cores), rows)
+    port_one, r2, e2 = rate(lambda: O.fft_rows_f32(few, -1, nthreads=1), few.shape[0])
+    line = {"value": round(port_all, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"oracle recursive radix-4/2 FFT in float, fft Forward over the full {rows}x{n} "
-                      f"Complex Float array, {reps} reps in {el:.1f} s on {cores} threads"}
+                      f"Complex Float array (seed 2), {r1} reps in {e1:.1f} s on {cores} threads",
+            "port_1_thread": {"value": round(port_one, 2), "unit": "GFLOP/s", "cores": 1,
+                              "sample": f"1024 of the rows, {r2} reps in {e2:.1f} s"}}
+    try:
+        import scipy
+        import scipy.fft as sf
+        z = few[:1].copy()
+        t0 = time.perf_counter()
+        sf.fft(z, axis=-1, workers=1)  # first call at this length: builds the plan
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(20):
+            sf.fft(z, axis=-1, workers=1)
+        steady = (time.perf_counter() - t0) / 20
+        s_all, r3, e3 = rate(lambda: sf.fft(full, axis=-1, workers=cores), rows)
+        s_one, r4, e4 = rate(lambda: sf.fft(few, axis=-1, workers=1), few.shape[0])
+        line["substitute"] = {
+            "label": "substitute, not the reference: scipy.fft (pocketfft) " + scipy.__version__
+                     + ", complex64, unnormalised forward, out of place (FFTW is not installed)",
+            "all_cores": {"value": round(s_all, 2), "unit": "GFLOP/s", "cores": cores,
+                          "sample": f"full {rows}x{n}, {r3} reps in {e3:.1f} s"},
+            "one_thread": {"value": round(s_one, 2), "unit": "GFLOP/s", "cores": 1,
+                           "sample": f"1024 rows, {r4} reps in {e4:.1f} s"},
+            "plan_us": round(max(0.0, first - steady) * 1e6, 1)}
+    except ImportError as e:
+        line["substitute"] = {"unavailable": str(e)}
+    return line
 
 
-def ragged_extras(afft, Mode, torch, dev, args):
+# ----------------------------------------------------------------------------------------------
+# Workloads beyond the headline
+
+def ragged_extras(afft, Mode, torch, dev, args, chk):
     """10,000 Complex Float transforms of mixed lengths through one ragged plan (the lengths that fit
     one workgroup in one persistent launch, the longer ones as batched plans over their gathered
     rows; eager and as a replayed hipGraph) against one plain call per transform and one batched
@@ -134,13 +285,15 @@ def ragged_extras(afft, Mode, torch, dev, args):
     import ctypes
     lengths = ragged_lengths()
     total = sum(lengths)
-    x = uniform_c((total,), torch.complex64, 4, dev)
+    xh = splitmix_uniform(4, (total,), "complex64")
+    x = to_device(xh, dev)
     out = torch.empty_like(x)
     plan = afft.RaggedPlan(lengths, torch.complex64)
     flops = sum(5.0 * n * math.log2(n) for n in lengths)
     res = {"ragged_total_elements": total}
     t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(5, args.steps // 2), 3)
     tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(5, args.steps // 2), 3)
+    got = out.cpu().numpy()
     # plain: one fft1D call per transform, straight through the C ABI (no tensor allocation)
     lib = afft._lib.lib()
     offs = [0]
@@ -183,7 +336,9 @@ def ragged_extras(afft, Mode, torch, dev, args):
     del idx
     res["ragged_persistent"] = {"us": round(t * 1e6, 1), "gflops": round(flops / t / 1e9, 1),
                                 "hbm_frac": round(16.0 * total / t / 1e9 / HBM_PEAK_GBS, 4)}
-    res["ragged_persistent_graph"] = {"us": round(tg * 1e6, 1), "gflops": round(flops / tg / 1e9, 1)}
+    res["ragged_persistent_graph"] = {"us": round(tg * 1e6, 1), "gflops": round(flops / tg / 1e9, 1),
+                                      "hbm_frac": round(16.0 * total / tg / 1e9 / HBM_PEAK_GBS, 4)}
+    res["ragged_persistent_graph"].update(chk.ragged(got, xh, lengths, Mode.Forward))
     res["ragged_plain_per_call"] = {"us": round(tp * 1e6, 1), "calls": len(lengths)}
     res["ragged_per_length_batched"] = {"us": round(tl * 1e6, 1), "calls": len(by)}
     res["ragged_per_length_packed"] = {"us": round(tlp * 1e6, 1), "calls": len(by),
@@ -195,26 +350,27 @@ def ragged_extras(afft, Mode, torch, dev, args):
     return res
 
 
-def spectral_extras(afft, Mode, torch, dev, args):
+def spectral_extras(afft, Mode, torch, dev, args, chk):
     """y = Inverse(fft2D(x) * H) over 64 x 512 x 512 Complex Float, forward + VJP: the fused
     kernels (the VJP recomputing F x, and the forward storing it for the VJP) against the plain
     per-call composition (one fft2D per image, separate product, conjugate-product, batch-sum and
     scale kernels)."""
     B, h, w = 64, 512, 512
-    x = normal_c((B, h, w), torch.complex64, 5, dev)
-    H = normal_c((h, w), torch.complex64, 6, dev)
-    t = normal_c((B, h, w), torch.complex64, 7, dev)
+    xh = splitmix_normal(5, (B, h, w), "complex64")
+    Hh = splitmix_normal(6, (h, w), "complex64")
+    th = splitmix_normal(7, (B, h, w), "complex64")
+    x, H, t = to_device(xh, dev), to_device(Hh, dev), to_device(th, dev)
     M = h * w
 
     def fused():
         y = afft.spectral_conv(x, H)
         gy = 2 * (y - t)
-        return afft.spectral_conv_vjp(x, H, gy)
+        return y, afft.spectral_conv_vjp(x, H, gy)
 
     def fused_saved():  # the forward stores F x, the VJP reads it back
         y, spec = afft.spectral_conv(x, H, save_spectrum=True)
         gy = 2 * (y - t)
-        return afft.spectral_conv_vjp(None, H, gy, spectrum=spec)
+        return y, afft.spectral_conv_vjp(None, H, gy, spectrum=spec)
 
     def plain():
         ys = []
@@ -234,43 +390,56 @@ def spectral_extras(afft, Mode, torch, dev, args):
     tfs = time_calls(fused_saved, max(3, args.steps // 10), 2)
     tp = time_calls(plain, 3, 1)
     tfwd = time_calls(lambda: afft.spectral_conv(x, H), max(3, args.steps // 5), 2)
-    return {"spectral_conv_fused_fwd_vjp": {"us": round(tf * 1e6, 1)},
-            "spectral_conv_fused_fwd_only": {"us": round(tfwd * 1e6, 1),
-                                             "hbm_frac_per_pass_3": round(3 * 2 * 8.0 * B * M / tfwd / 1e9 / HBM_PEAK_GBS, 4)},
-            "spectral_conv_fused_fwd_vjp_saved": {"us": round(tfs * 1e6, 1),
-                                                  "note": "forward stores F x, VJP reads it"},
-            "spectral_conv_plain_fwd_vjp": {"us": round(tp * 1e6, 1)},
-            "spectral_conv_speedup_fwd_vjp": round(tp / tf, 2),
-            "spectral_conv_speedup_fwd_vjp_saved": round(tp / tfs, 2)}
+    res = {"spectral_conv_fused_fwd_vjp": {"us": round(tf * 1e6, 1)},
+           "spectral_conv_fused_fwd_only": {"us": round(tfwd * 1e6, 1),
+                                            "hbm_frac_per_pass_3": round(3 * 2 * 8.0 * B * M / tfwd / 1e9 / HBM_PEAK_GBS, 4)},
+           "spectral_conv_fused_fwd_vjp_saved": {"us": round(tfs * 1e6, 1),
+                                                 "note": "forward stores F x, VJP reads it"},
+           "spectral_conv_plain_fwd_vjp": {"us": round(tp * 1e6, 1)},
+           "spectral_conv_speedup_fwd_vjp": round(tp / tf, 2),
+           "spectral_conv_speedup_fwd_vjp_saved": round(tp / tfs, 2)}
+    for key, fn in (("spectral_conv_fused_fwd_vjp", fused), ("spectral_conv_fused_fwd_vjp_saved", fused_saved)):
+        y, (gx, gH) = fn()
+        res[key].update(chk.spectral(xh, Hh, th, y.cpu().numpy(), gx.cpu().numpy(), gH.cpu().numpy()))
+    gxp, gHp = plain()
+    yp = torch.stack([afft.fft2D(Mode.Inverse, afft.fft2D(Mode.Forward, x[b]) * H) for b in range(B)])
+    res["spectral_conv_plain_fwd_vjp"].update(
+        chk.spectral(xh, Hh, th, yp.cpu().numpy(), gxp.cpu().numpy(), gHp.cpu().numpy()))
+    return res
 
 
 def _prof_fft2d(afft, Mode, torch, dev):
-    x = normal_c((4096, 4096), torch.complex64, 3, dev)
+    x = to_device(splitmix_normal(3, (4096, 4096), "complex64"), dev)
     return lambda: afft.fft2D(Mode.Forward, x)
 
 
 def _prof_fft3d(afft, Mode, torch, dev):
-    x = normal_c((256, 256, 256), torch.complex128, 3, dev)
+    x = to_device(splitmix_normal(3, (256, 256, 256), "complex128"), dev)
     return lambda: afft.fft3D(Mode.Forward, x)
 
 
 def _prof_four_step(afft, Mode, torch, dev):
-    x = uniform_c((1, 1 << 22), torch.complex64, 2, dev)
+    x = to_device(splitmix_uniform(2, (1, 1 << 22), "complex64"), dev)
     return lambda: afft.fft(Mode.Forward, x)
+
+
+def _prof_batched2d(afft, Mode, torch, dev):
+    x = to_device(splitmix_normal(5, (64, 512, 512), "complex64"), dev)
+    return lambda: afft.fft2D_many(Mode.Forward, x)
 
 
 def _prof_ragged(afft, Mode, torch, dev):
     lengths = ragged_lengths()
-    x = uniform_c((sum(lengths),), torch.complex64, 4, dev)
+    x = to_device(splitmix_uniform(4, (sum(lengths),), "complex64"), dev)
     out = torch.empty_like(x)
     plan = afft.RaggedPlan(lengths, torch.complex64)
     return lambda: plan.execute(Mode.Forward, x, out=out, graph=True)
 
 
 def _prof_spectral(afft, Mode, torch, dev):
-    x = normal_c((64, 512, 512), torch.complex64, 5, dev)
-    H = normal_c((512, 512), torch.complex64, 6, dev)
-    t = normal_c((64, 512, 512), torch.complex64, 7, dev)
+    x = to_device(splitmix_normal(5, (64, 512, 512), "complex64"), dev)
+    H = to_device(splitmix_normal(6, (512, 512), "complex64"), dev)
+    t = to_device(splitmix_normal(7, (64, 512, 512), "complex64"), dev)
 
     def call():
         y = afft.spectral_conv(x, H)
@@ -279,7 +448,7 @@ def _prof_spectral(afft, Mode, torch, dev):
 
 
 PROFILE_WORKLOADS = {"fft2d": _prof_fft2d, "fft3d": _prof_fft3d, "four_step": _prof_four_step,
-                     "ragged": _prof_ragged, "spectral": _prof_spectral}
+                     "batched2d": _prof_batched2d, "ragged": _prof_ragged, "spectral": _prof_spectral}
 
 
 def main():
@@ -311,7 +480,8 @@ def main():
         return
 
     rows, n = 16384, 4096
-    x = uniform_c((rows, n), torch.complex64, 2, dev)
+    xh = splitmix_uniform(2, (rows, n), "complex64")
+    x = to_device(xh, dev)
     out = torch.empty_like(x)
     plan = afft.Plan((rows, n), torch.complex64, "innermost")
     nl = plan.info()["launches"]
@@ -347,6 +517,7 @@ def main():
     flops = 5.0 * n * math.log2(n) * rows
     alg_bytes = 2.0 * rows * n * 8
     achieved = alg_bytes / kern / 1e9
+    chk = Checker()
     line = {
         "metric": METRIC,
         "value": round(flops / wall / 1e9, 1),
@@ -359,7 +530,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: re/im uniform[-1,1) from torch.rand seed 2, resident in HBM",
+        "data": "synthetic: re/im uniform[-1,1) from the SplitMix64 stream of seed 2, resident in HBM",
         "config": {"workload": "fft Forward along the innermost axis of Array DIM2 (Complex Float) 16384x4096",
                    "rows": rows, "n": n, "mode": "Forward", "launches_per_step": nl},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -367,56 +538,74 @@ def main():
                      "traffic": read_traffic("afft_rows_f32_4096_fft_16384x4096.txt"),
                      "kernel_avg_us": round(kern * 1e6, 2)},
     }
+    line["accuracy"] = chk.entry(out.cpu().numpy(), chk.O.fft(chk.O.Mode.Forward, xh), False, n)
     del plan
 
     extras = {}
     if not args.no_extras:
-        def add(name, fn, n_total, bytes_alg, flops_call):
+        def add(name, fn, bytes_alg, flops_call, check=None):
             try:
                 t = time_calls(fn, max(5, args.steps // 2), 3)
                 extras[name] = {"us": round(t * 1e6, 2), "gflops": round(flops_call / t / 1e9, 1),
                                 "hbm_frac": round(bytes_alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+                if check is not None:
+                    extras[name].update(check(fn()))
             except afft.AfftError as e:
                 extras[name] = {"unavailable": str(e)}
 
+        def against(want_fn, xin, f64, nn):
+            return lambda y: chk.entry(y.cpu().numpy(), want_fn(xin), f64, nn)
+
         for mode in (Mode.Reverse, Mode.Inverse):
-            add(f"fft_16384x4096_c64_{mode.name}", lambda m=mode: afft.fft(m, x), rows * n, alg_bytes, flops)
+            add(f"fft_16384x4096_c64_{mode.name}", lambda m=mode: afft.fft(m, x), alg_bytes, flops,
+                against(lambda a, m=mode: chk.O.fft(chk.mode(m), a), xh, False, n))
         del x, out
         torch.cuda.empty_cache()
         # the reference's CPU-runnable case: fft1D Complex Double, N = 1024, Forward then Inverse
-        # (latency-bound: two launches of one workgroup each)
-        v = uniform_c((1024,), torch.complex128, 0, dev)
+        # (latency-bound: two launches of one workgroup each); checked as the round trip
+        vh = splitmix_uniform(0, (1024,), "complex128")
+        v = to_device(vh, dev)
         add("fft1D_1024_c128_forward_then_inverse",
-            lambda: afft.fft1D(Mode.Inverse, afft.fft1D(Mode.Forward, v)), 1024, 2 * 32.0 * 1024,
-            2 * 5.0 * 1024 * 10)
+            lambda: afft.fft1D(Mode.Inverse, afft.fft1D(Mode.Forward, v)), 2 * 32.0 * 1024,
+            2 * 5.0 * 1024 * 10, against(lambda a: a, vh, True, 1024))
+        extras["fft1D_1024_c128_forward"] = chk.entry(afft.fft1D(Mode.Forward, v).cpu().numpy(),
+                                                      chk.O.fft1D(chk.O.Mode.Forward, vh), True, 1024)
         del v
-        big = uniform_c((1, 1 << 22), torch.complex64, 2, dev)
         N = 1 << 22
-        add("fft_1x4194304_c64_four_step", lambda: afft.fft(Mode.Forward, big), N, 16.0 * N, 5.0 * N * 22)
+        bh = splitmix_uniform(2, (1, N), "complex64")
+        big = to_device(bh, dev)
+        for mode in (Mode.Forward, Mode.Reverse, Mode.Inverse):
+            add(f"fft_1x4194304_c64_four_step_{mode.name}", lambda m=mode: afft.fft(m, big), 16.0 * N, 5.0 * N * 22,
+                against(lambda a, m=mode: chk.O.fft(chk.mode(m), a), bh, False, N))
         del big
-        m2 = normal_c((4096, 4096), torch.complex64, 3, dev)
+        mh = splitmix_normal(3, (4096, 4096), "complex64")
+        m2 = to_device(mh, dev)
         N = 4096 * 4096
-        add("fft2D_4096x4096_c64_Forward", lambda: afft.fft2D(Mode.Forward, m2), N, 16.0 * N, 5.0 * N * 24)
-        add("fft2D_4096x4096_c64_Inverse", lambda: afft.fft2D(Mode.Inverse, m2), N, 16.0 * N, 5.0 * N * 24)
+        for mode in (Mode.Forward, Mode.Inverse):
+            add(f"fft2D_4096x4096_c64_{mode.name}", lambda m=mode: afft.fft2D(m, m2), 16.0 * N, 5.0 * N * 24,
+                against(lambda a, m=mode: chk.O.fft2D(chk.mode(m), a), mh, False, N))
         del m2
         # batched 2D (the spectral-convolution batch): 64 images of 512 x 512, two passes
-        mb = normal_c((64, 512, 512), torch.complex64, 5, dev)
+        bh2 = splitmix_normal(5, (64, 512, 512), "complex64")
+        mb = to_device(bh2, dev)
         N = 64 * 512 * 512
-        add("fft2D_many_64x512x512_c64_Forward", lambda: afft.fft2D_many(Mode.Forward, mb), N, 16.0 * N,
-            5.0 * N * 18)
+        add("fft2D_many_64x512x512_c64_Forward", lambda: afft.fft2D_many(Mode.Forward, mb), 16.0 * N,
+            5.0 * N * 18, against(lambda a: chk.O.fft_axes(a, (1, 2), -1), bh2, False, 512 * 512))
         del mb
-        c3 = normal_c((256, 256, 256), torch.complex128, 3, dev)
+        ch = splitmix_normal(3, (256, 256, 256), "complex128")
+        c3 = to_device(ch, dev)
         N = 256 ** 3
-        add("fft3D_256x256x256_c128_Forward", lambda: afft.fft3D(Mode.Forward, c3), N, 32.0 * N, 5.0 * N * 24)
+        for mode in (Mode.Forward, Mode.Inverse):
+            add(f"fft3D_256x256x256_c128_{mode.name}", lambda m=mode: afft.fft3D(m, c3), 32.0 * N, 5.0 * N * 24,
+                against(lambda a, m=mode: chk.O.fft3D(chk.mode(m), a), ch, True, N))
         del c3
         torch.cuda.empty_cache()
-        extras.update(ragged_extras(afft, Mode, torch, dev, args))
-        extras.update(spectral_extras(afft, Mode, torch, dev, args))
+        extras.update(ragged_extras(afft, Mode, torch, dev, args, chk))
+        extras.update(spectral_extras(afft, Mode, torch, dev, args, chk))
         line["extras"] = extras
 
     if not args.no_cpu:
-        cb = cpu_baseline(rows, n)
-        line["cpu_baseline"] = cb
+        line["cpu_baseline"] = cpu_baseline(xh, rows, n)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
