@@ -7,16 +7,17 @@
 // resource sets per plan and leases one to each execution:
 //
 //   * an idle set last used on the caller's stream is taken first: the stream itself orders the
-//     two executions (the event wait below is a no-op within one stream);
-//   * else an idle set whose last execution has finished on the device;
+//     two executions, so nothing is recorded or waited on (a stream that keeps one plan busy pays
+//     no synchronisation at all);
+//   * else an unused set, or one whose recorded last execution has finished on the device;
 //   * else a new set, up to kMaxLeases;
-//   * else an idle set still in flight on another stream, with the caller's stream made to wait
-//     on the event recorded after that execution (correct, only less concurrent);
+//   * else an idle set still in flight on another stream: an event is recorded on that stream
+//     (after the executions that used the set) and the caller's stream waits on it (correct,
+//     only less concurrent);
 //   * else (every set is being enqueued by another host thread) the caller waits for one.
 //
-// After the execution is enqueued, the lease is returned with an event recorded on the caller's
-// stream.  So executions on different streams never share a set while either is in flight, and a
-// single stream keeps reusing one set.  This settles the open question of the cuFFT plugin,
+// So executions on different streams never share a set while either is in flight, and a single
+// stream keeps reusing one set.  This settles the open question of the cuFFT plugin,
 // whose handle carries the stream and the work area together (LLVM/PTX.hs:121-124,
 // PTX/Plans.hs:58-63).
 //
@@ -39,13 +40,23 @@ namespace afft {
 
 constexpr int kMaxLeases = 4;
 
+// true when s is recording a graph capture (its work is replayed later, not run now)
+inline bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return false;
+    return cs == hipStreamCaptureStatusActive;
+}
+
+
 template <class R> class ExecPool {
 public:
     struct Entry {
         R res{};
-        hipEvent_t done = nullptr;    // recorded after the last execution that used res
-        hipStream_t last = nullptr;   // the stream that execution ran on
-        bool recorded = false;
+        hipEvent_t done = nullptr;    // recorded after the executions that used res (see dirty)
+        hipStream_t last = nullptr;   // the stream of the last execution that used res
+        bool used = false;            // some execution has used res
+        bool dirty = false;           // executions on `last` since `done` was recorded
         bool busy = false;            // leased, its execution not yet enqueued
         bool pinned = false;          // owned by a captured graph or retired after an error
     };
@@ -57,26 +68,30 @@ public:
     int acquire(hipStream_t s, const Make &make, Entry *&out)
     {
         out = nullptr;
+        const bool per_thread = s == hipStreamPerThread;
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             Entry *pick = nullptr;
             bool wait = false;
             int unpinned = 0;
-            for (auto &e : entries_) {
-                if (e->pinned) continue;
-                ++unpinned;
-                if (!e->busy && e->recorded && e->last == s) {
-                    pick = e.get();
-                    wait = true; // same stream: ordered already, the wait costs nothing
-                    break;
-                }
-            }
-            if (!pick)
+            for (auto &e : entries_)
+                if (!e->pinned) ++unpinned;
+            // 1. an idle set whose executions all went to this stream: the stream orders them
+            if (!per_thread)
                 for (auto &e : entries_)
-                    if (!e->pinned && !e->busy && (!e->recorded || hipEventQuery(e->done) == hipSuccess)) {
+                    if (!e->pinned && !e->busy && e->used && e->last == s) {
                         pick = e.get();
                         break;
                     }
+            // 2. a set never used, or one whose recorded last execution has finished
+            if (!pick)
+                for (auto &e : entries_)
+                    if (!e->pinned && !e->busy &&
+                        (!e->used || (!e->dirty && hipEventQuery(e->done) == hipSuccess))) {
+                        pick = e.get();
+                        break;
+                    }
+            // 3. a new set
             if (!pick && unpinned < kMaxLeases) {
                 std::unique_ptr<Entry> e(new (std::nothrow) Entry());
                 if (!e) return AFFT_ERROR_ALLOC;
@@ -89,17 +104,45 @@ public:
                 pick = e.get();
                 entries_.push_back(std::move(e));
             }
+            // 4. an idle set in flight on another stream: this stream waits for it on the device.
+            // Executions of a set on one stream are not recorded one by one; the event is recorded
+            // on that stream now, after them (never into a capture in progress there).
             if (!pick)
-                for (auto &e : entries_)
-                    if (!e->pinned && !e->busy) {
-                        pick = e.get();
-                        wait = true;
-                        break;
+                for (auto &e : entries_) {
+                    if (e->pinned || e->busy) continue;
+                    if (e->dirty) {
+                        if (stream_capturing(e->last)) continue;
+                        if (hipEventRecord(e->done, e->last) != hipSuccess) continue;
+                        e->dirty = false;
                     }
+                    pick = e.get();
+                    wait = true;
+                    break;
+                }
+            if (!pick) {
+                // every idle set sits on a stream that cannot take an event now (a capture in
+                // progress, or a stream that is gone): with none busy, nobody would wake this
+                // thread, so take one more set beyond the bound
+                bool any_busy = false;
+                for (auto &e : entries_)
+                    if (!e->pinned && e->busy) any_busy = true;
+                if (!any_busy) {
+                    std::unique_ptr<Entry> e(new (std::nothrow) Entry());
+                    if (!e) return AFFT_ERROR_ALLOC;
+                    if (hipEventCreateWithFlags(&e->done, hipEventDisableTiming) != hipSuccess) return AFFT_ERROR_HIP;
+                    const int st = make(e->res);
+                    if (st) {
+                        (void)hipEventDestroy(e->done);
+                        return st;
+                    }
+                    pick = e.get();
+                    entries_.push_back(std::move(e));
+                }
+            }
             if (pick) {
                 pick->busy = true;
                 lk.unlock();
-                if (wait && pick->recorded && hipStreamWaitEvent(s, pick->done, 0) != hipSuccess) {
+                if (wait && hipStreamWaitEvent(s, pick->done, 0) != hipSuccess) {
                     release(pick, nullptr, false);
                     return AFFT_ERROR_HIP;
                 }
@@ -130,17 +173,22 @@ public:
         return AFFT_SUCCESS;
     }
 
-    // Return a lease after its execution was enqueued on s (null s: nothing was enqueued).
-    // Pinned sets stay out of circulation.
+    // Return a lease after its execution was enqueued on s.  No event is recorded here: a set
+    // that stays on one stream never needs one, and another stream that wants the set records
+    // it on `last` then.  The per-thread default stream names a different stream in every
+    // thread, so an execution there records its event at once.  Pinned sets stay out of
+    // circulation.
     void release(Entry *e, hipStream_t s, bool enqueued)
     {
         if (!e) return;
         if (!e->pinned && enqueued) {
-            if (hipEventRecord(e->done, s) == hipSuccess) {
-                e->recorded = true;
-                e->last = s;
+            e->used = true;
+            e->last = s;
+            if (s == hipStreamPerThread) {
+                if (hipEventRecord(e->done, s) == hipSuccess) e->dirty = false;
+                else e->pinned = true; // its last use cannot be tracked: retire it
             } else {
-                e->pinned = true; // its last use cannot be tracked: retire it
+                e->dirty = true;
             }
         }
         {
@@ -185,13 +233,5 @@ private:
     std::condition_variable cv_;
     std::vector<std::unique_ptr<Entry>> entries_;
 };
-
-// true when s is recording a graph capture (its work is replayed later, not run now)
-inline bool stream_capturing(hipStream_t s)
-{
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return false;
-    return cs == hipStreamCaptureStatusActive;
-}
 
 } // namespace afft
