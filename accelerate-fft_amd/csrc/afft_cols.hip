@@ -13,6 +13,8 @@
 // lengths, each one HBM read and one HBM write of the array with whole lines.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "../../include/afft.h"
 #include "afft_device.hpp"
 #include "afft_internal.hpp"
@@ -77,6 +79,148 @@ afft_regcol(const FftArgs a)
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// The same transforms with each column spread over Q = 2 or 4 neighbouring lanes of a quad
+// (afft_regcol_q): Q times the waves of afft_regcol, each holding R/Q elements, so loads of later
+// waves overlap the arithmetic and stores of earlier ones.  With M = R/Q, lane q of a column holds
+// x[q + Q m] (m < M), and the transform is the two-level split of index i = q + Q m,
+// k = a + M b:
+//   Y_q[a] = W_R^{q a} * DFT_M(x[q + Q m])[a]                     (in the lane's registers)
+//   X[a + M b] = sum_q W_Q^{q b} Y_q[a]                            (across the Q lanes)
+// The cross-lane step exchanges Q x Q blocks of Y (a = Q s + r) with DPP quad permutations, so
+// that lane l holds Y_q[Q s + l] for every q, and then runs DFT_Q in registers: lane l ends
+// with X[Q s + l + M b] for s < M/Q, b < Q.  The lanes of a wave still read and write whole
+// lines: a wave instruction covers 64/Q columns on Q neighbouring rows.
+
+// lane l receives the value lane l ^ X of its quad holds (DPP quad_perm, a VALU operation)
+template <int X> __device__ __forceinline__ int quad_xor_i(int v)
+{
+    static_assert(X == 1 || X == 2, "quad lanes");
+    // quad_perm selectors: lane i of each quad reads lane ((ctrl >> 2i) & 3)
+    constexpr int ctrl = X == 1 ? 0xB1 /* 1,0,3,2 */ : 0x4E /* 2,3,0,1 */;
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, false);
+}
+
+template <int X> __device__ __forceinline__ float quad_xor(float v)
+{
+    return __builtin_bit_cast(float, quad_xor_i<X>(__builtin_bit_cast(int, v)));
+}
+
+template <int X> __device__ __forceinline__ double quad_xor(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = quad_xor_i<X>((int)(b & 0xffffffffll));
+    const int hi = quad_xor_i<X>((int)(b >> 32));
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+template <int X, typename V> __device__ __forceinline__ V quad_xor_c(V v)
+{
+    return c_make<V>(quad_xor<X>(v.x), quad_xor<X>(v.y));
+}
+
+// Transpose of the Q x Q block (lane l of the quad group, registers y[0..Q-1]) across the lanes:
+// after it lane l holds what lane r held in register l, in register r.  One stage per lane bit:
+// element (l, r) with bit B of l and r different trades places with (l ^ B, r ^ B).
+template <int Q, int B, typename V> __device__ __forceinline__ void quad_transpose_stage(V *y, int l)
+{
+    V p[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) p[r] = quad_xor_c<B>(y[r ^ B]);
+#pragma unroll
+    for (int r = 0; r < Q; ++r)
+        if (((l & B) != 0) != ((r & B) != 0)) y[r] = p[r];
+}
+
+template <int Q, typename V> __device__ __forceinline__ void quad_transpose(V *y, int l)
+{
+    quad_transpose_stage<Q, 1>(y, l);
+    if constexpr (Q == 4) quad_transpose_stage<Q, 2>(y, l);
+}
+
+// v * W_R^{q a} for the lane's q < Q (a compile-time, q at run time)
+template <int R, int Q, int A, typename V> __device__ __forceinline__ V lane_twiddle(V v, int q)
+{
+    const V t1 = twiddle_const<R, A>(v);
+    if constexpr (Q == 2) return q ? t1 : v;
+    const V t2 = twiddle_const<R, 2 * A>(v);
+    const V t3 = twiddle_const<R, 3 * A>(v);
+    return q == 0 ? v : q == 1 ? t1 : q == 2 ? t2 : t3;
+}
+
+template <int R, int Q, typename V, int... As>
+__device__ __forceinline__ void lane_twiddles(V *y, int q, std::integer_sequence<int, As...>)
+{
+    ((y[As] = lane_twiddle<R, Q, As>(y[As], q)), ...);
+}
+
+template <typename Real, int R, int Q>
+__global__ void __launch_bounds__(kRegColBlock) afft_regcol_q(const FftArgs a)
+{
+    using V = typename cplx_of<Real>::type;
+    constexpr int M = R / Q;
+    static_assert(M % Q == 0, "Q x Q blocks");
+    const int64_t tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int q = threadIdx.x % Q;
+    const int64_t j = tile * (kRegColBlock / Q) + threadIdx.x / Q;
+    // the Q lanes of a column stay or leave together, so the quad exchanges below only ever read
+    // lanes of the same column
+    if (j >= a.ntrans) return;
+
+    const int64_t ibase = addr_base(a.ain, j);
+    const V *in = reinterpret_cast<const V *>(a.in) + ibase;
+    V y[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const V v = c_load(in + (int64_t)(q + Q * m) * a.ain.es, (bool)a.nt_in);
+        y[m] = a.conj_in ? c_conj(v) : v;
+    }
+    if (a.sg_in.on) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) y[m] = sign_apply(a.sg_in, ibase + (int64_t)(q + Q * m) * a.ain.es, y[m]);
+    }
+    dft<M>::run(y);
+    lane_twiddles<R, Q>(y, q, std::make_integer_sequence<int, M>{});
+#pragma unroll
+    for (int s = 0; s < M / Q; ++s) {
+        quad_transpose<Q>(y + Q * s, q);
+        dft<Q>::run(y + Q * s); // lane q: y[Q s + b] = X[Q s + q + M b]
+    }
+
+    // output k of register Q s + b
+    auto kidx = [&](int s, int b) { return (int64_t)(Q * s + q + M * b); };
+    if (a.post_L) {
+        // four-step twiddle W_L^{jl k}, k = q + Q s + M b: W^{jl q} W^{jl Q s} W^{jl M b}
+        const V *hi = reinterpret_cast<const V *>(a.post_hi);
+        const V *lo = reinterpret_cast<const V *>(a.post_lo);
+        const int64_t jl = addr_level(a.ain, j, a.post_level);
+        const V wq = power_lookup(hi, lo, jl * q, a.post_shift);
+        V wb[Q];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) wb[b] = power_lookup(hi, lo, jl * (int64_t)(M * b), a.post_shift);
+#pragma unroll
+        for (int s = 0; s < M / Q; ++s) {
+            const V ws = c_mul(wq, power_lookup(hi, lo, jl * (int64_t)(Q * s), a.post_shift));
+#pragma unroll
+            for (int b = 0; b < Q; ++b) y[Q * s + b] = c_mul(y[Q * s + b], c_mul(ws, wb[b]));
+        }
+    }
+
+    const int64_t obase = addr_base(a.aout, j);
+    V *out = reinterpret_cast<V *>(a.out) + obase;
+    const Real sc = (Real)a.scale;
+#pragma unroll
+    for (int s = 0; s < M / Q; ++s) {
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            V v = y[Q * s + b];
+            if (a.sg_out.on) v = sign_apply(a.sg_out, obase + kidx(s, b) * a.aout.es, v);
+            if (a.conj_out) v = c_conj(v);
+            c_store(out + kidx(s, b) * a.aout.es, c_scale(v, sc), (bool)a.nt_out);
+        }
+    }
+}
+
 bool regcol_supported(int dtype, int64_t n)
 {
     // FP64 stops at 32: 64 FP64 complex values alone would take 256 VGPRs
@@ -88,6 +232,19 @@ int regcol_block() { return kRegColBlock; }
 template <typename Real>
 static hipError_t launch_regcol_t(const FftArgs &a, int64_t grid, hipStream_t s)
 {
+    if (a.lanes == 2 || a.lanes == 4) {
+        const int key = a.E * 8 + a.lanes;
+        switch (key) {
+        case 16 * 8 + 2: hipLaunchKernelGGL((afft_regcol_q<Real, 16, 2>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        case 16 * 8 + 4: hipLaunchKernelGGL((afft_regcol_q<Real, 16, 4>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        case 32 * 8 + 2: hipLaunchKernelGGL((afft_regcol_q<Real, 32, 2>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        case 32 * 8 + 4: hipLaunchKernelGGL((afft_regcol_q<Real, 32, 4>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        case 64 * 8 + 2: hipLaunchKernelGGL((afft_regcol_q<Real, 64, 2>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        case 64 * 8 + 4: hipLaunchKernelGGL((afft_regcol_q<Real, 64, 4>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.E) {
     case 16: hipLaunchKernelGGL((afft_regcol<Real, 16>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;
     case 32: hipLaunchKernelGGL((afft_regcol<Real, 32>), dim3((unsigned)grid), dim3(kRegColBlock), 0, s, a); break;

@@ -110,6 +110,7 @@ struct FftArgs {
     int xcd_remap;       // map workgroups to tiles XCD-contiguously (strided passes)
     int nt_in, nt_out;   // streaming loads / stores (plan input read once / final output)
     SignSpec sg_in, sg_out; // centring signs on load (first pass) / on store (last pass)
+    int lanes;           // register kernel: lanes per transform (1; 2 or 4 exchange through DPP)
 };
 
 // Fused single-workgroup Bluestein (chirp-z) launch for a length n that is not 5-smooth:

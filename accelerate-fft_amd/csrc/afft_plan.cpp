@@ -210,6 +210,19 @@ struct Builder {
         return buf;
     }
 
+    // Lanes per transform of the register kernel: AFFT_REGCOL_Q (1, 2 or 4) for measurements,
+    // else the default.
+    static int regcol_lanes(int dtype, int64_t n)
+    {
+        (void)dtype;
+        (void)n;
+        if (const char *v = std::getenv("AFFT_REGCOL_Q")) {
+            const int q = std::atoi(v);
+            if (q == 1 || q == 2 || q == 4) return q;
+        }
+        return 1;
+    }
+
     // Columns a strided workgroup aims for: neighbouring lanes take neighbouring columns, so C
     // columns make one C * element-size byte segment per row; at least 16 columns (a 128-byte
     // line in FP32), more while the workgroup stays at or under 256 threads.
@@ -293,9 +306,10 @@ struct Builder {
             a.post_hi = post_hi;
             a.post_lo = post_lo;
             a.post_shift = post_shift;
+            a.lanes = regcol_lanes(plan.dtype, n);
             l.block = regcol_block();
             l.lds = 0;
-            l.grid = (ntrans + l.block - 1) / l.block;
+            l.grid = (ntrans * a.lanes + l.block - 1) / l.block;
             plan.launches.push_back(l);
             return;
         }
