@@ -46,6 +46,28 @@ struct ColConvArgs {
     int other_is_spec;   // VJP: `other` holds F x (column transform already done)
 };
 
+// Workgroup -> (outer index, column tile).  The dispatcher deals workgroup b to XCD b mod 8; each
+// XCD takes a fixed eighth of the column tiles for every image (or image group), so it reads only
+// its eighth of the filter H, which then stays in that XCD's L2 from image to image instead of
+// being fetched again for every image (measured: forward 171 -> 164 us on 64 x 512^2; the VJP,
+// bound elsewhere, unchanged).  Falls back to the XCD-contiguous tile order when the
+// column tiles do not split evenly.
+__device__ __forceinline__ void colconv_tile(int64_t tiles, int64_t outer, int64_t &o, int64_t &ct)
+{
+    constexpr int kXcd = 8;
+    const int64_t b = blockIdx.x;
+    if (tiles % kXcd == 0 && (int64_t)gridDim.x == tiles * outer) {
+        const int64_t per = tiles / kXcd;
+        const int64_t k = b / kXcd;
+        o = k / per;
+        ct = (b % kXcd) * per + k % per;
+        return;
+    }
+    const int64_t tile = xcd_tile(b, gridDim.x);
+    o = tile / tiles;
+    ct = tile % tiles;
+}
+
 // The column transform of a tile: h = N with compile-time radices Rs (the Stockham passes of the
 // row kernels, twiddle bases loaded once per workgroup), or, for N = 0, the generic transform
 // of any 5-smooth h driven by the pass table.
@@ -78,9 +100,9 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     const int T = N ? N / E : (int)(a.h / E), C = a.C;
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int64_t tiles = (a.w + C - 1) / C;
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t b = tile / tiles;
-    const int64_t j = (tile % tiles) * C + c;
+    int64_t b, ct;
+    colconv_tile(tiles, a.B, b, ct);
+    const int64_t j = ct * C + c;
     const bool active = j < a.w && b < a.B;
     V *d = reinterpret_cast<V *>(a.data) + b * a.h * a.w + j;
     const V *Hc = reinterpret_cast<const V *>(a.H) + j;
@@ -122,9 +144,9 @@ __global__ void __launch_bounds__(vjp_block(sizeof(Real) == 8)) __attribute__((a
     const int T = N ? N / E : (int)(a.h / E), C = a.C;
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int64_t tiles = (a.w + C - 1) / C;
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t g = tile / tiles;
-    const int64_t j = (tile % tiles) * C + c;
+    int64_t g, ct;
+    colconv_tile(tiles, a.groups, g, ct);
+    const int64_t j = ct * C + c;
     const bool active = j < a.w;
     const int64_t per = (a.B + a.groups - 1) / a.groups;
     const int64_t b0 = g * per, b1 = (b0 + per < a.B) ? b0 + per : a.B;
@@ -464,7 +486,7 @@ static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *
         a.w = w;
         a.C = cs.C;
         a.groups = groups;
-        const int64_t grid = (int64_t)groups * tiles;
+            const int64_t grid = (int64_t)groups * tiles;
         hipError_t e = dtype == AFFT_C64 ? launch_colconv<float>(true, cs.L.E, a, cs.block, grid, cs.lds, s)
                                          : launch_colconv<double>(true, cs.L.E, a, cs.block, grid, cs.lds, s);
         if (e != hipSuccess) st = AFFT_ERROR_HIP;

@@ -33,7 +33,9 @@ One step = one call.
   --profile-headline runs only the headline plan (and a plain device copy of the same array,
              the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
   --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral,
-             batched2d) for warmup + steps calls, for profiling.
+             batched2d) for profiling.
+             Both first run untimed calls for 0.3 s (settle_calls in their output line), then
+             warmup + steps calls: a profile's steady state is its last `steps` launches.
 """
 import argparse
 import json
@@ -471,12 +473,25 @@ def main():
     from afft import Mode
 
     dev = torch.device("cuda", 0)
+    def settle(call, seconds=0.3):
+        """untimed calls until the clocks have ramped up; returns how many ran"""
+        n, end = 0, time.perf_counter() + seconds
+        while time.perf_counter() < end:
+            call()
+            n += 1
+            if n % 8 == 0:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        return n
+
     if args.profile_workload:
         call = PROFILE_WORKLOADS[args.profile_workload](afft, Mode, torch, dev)
+        ns = settle(call)
         for _ in range(args.warmup + args.steps):
             call()
         torch.cuda.synchronize()
-        print(json.dumps({"profile_workload": args.profile_workload, "steps": args.steps}), flush=True)
+        print(json.dumps({"profile_workload": args.profile_workload, "settle_calls": ns, "steps": args.steps}),
+              flush=True)
         return
 
     rows, n = 16384, 4096
@@ -490,12 +505,13 @@ def main():
         plan.execute(x, -1, 1.0, out=out)
 
     if args.profile_headline:
+        ns = settle(step)
         for _ in range(args.warmup + args.steps):
             step()
         for _ in range(args.steps):
             out.copy_(x)  # plain copy of the same 512 MiB: counter calibration
         torch.cuda.synchronize()
-        print(json.dumps({"profile_headline": True, "steps": args.steps}), flush=True)
+        print(json.dumps({"profile_headline": True, "settle_calls": ns, "steps": args.steps}), flush=True)
         return
 
     # settle: untimed steps until 50 ms have passed, so that the clocks of a freshly started
