@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libafft.so")
+# AFFT_LIB names another build of the same library (the host-sanitised one, for instance)
+LIB_PATH = os.environ.get("AFFT_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libafft.so")
 
 SUCCESS = 0
 C64 = 0

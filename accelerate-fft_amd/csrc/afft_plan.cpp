@@ -33,6 +33,22 @@ namespace afft {
 static const int kRadices[] = {16, 8, 5, 4, 3, 2};
 
 static thread_local int tl_row_variant = -1;
+
+// Device allocations of the library go through device_alloc.  Failure injection for tests:
+// with AFFT_TEST_FAIL_ALLOC=k set, the k-th allocation (k >= 1) of the calling thread since the
+// last AllocCount reset (the start of a plan creation) fails as hipMalloc's would.
+static thread_local int64_t tl_alloc_count = 0;
+void reset_alloc_count() { tl_alloc_count = 0; }
+
+hipError_t device_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    ++tl_alloc_count;
+    if (const char *v = std::getenv("AFFT_TEST_FAIL_ALLOC")) {
+        if (std::atoll(v) == tl_alloc_count) return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
 void force_row_variant(int v) { tl_row_variant = v; }
 int forced_row_variant() { return tl_row_variant; }
 
@@ -181,7 +197,7 @@ struct Builder {
     {
         if (bytes <= 0) return nullptr;
         void *p = nullptr;
-        if (hipMalloc(&p, (size_t)bytes) != hipSuccess) {
+        if (device_alloc(&p, (size_t)bytes) != hipSuccess) {
             status = AFFT_ERROR_ALLOC;
             return nullptr;
         }
@@ -368,7 +384,7 @@ struct Builder {
         if (status) return false;
         check(gen_chirp(plan.dtype, cbuf, n, s));
         void *b64 = nullptr;
-        if (hipMalloc(&b64, (size_t)M * 16) != hipSuccess) {
+        if (device_alloc(&b64, (size_t)M * 16) != hipSuccess) {
             status = AFFT_ERROR_ALLOC;
             return false;
         }
@@ -985,7 +1001,7 @@ int exec_plan(const Plan &plan, int direction, const void *d_in, void *d_out, do
     // lease this execution's own workspace (one per stream in flight, see afft_pool.hpp)
     auto make = [wbytes](void *&mem) {
         mem = nullptr;
-        return hipMalloc(&mem, (size_t)wbytes) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_ALLOC;
+        return device_alloc(&mem, (size_t)wbytes) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_ALLOC;
     };
     WorkspacePool::Entry *e = nullptr;
     const bool capturing = stream_capturing(s);
@@ -1154,6 +1170,7 @@ struct DeviceGuard {
 
 int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device)
 {
+    reset_alloc_count();
     DeviceInfo dev;
     int st = device_info(device, dev);
     if (st) return st;

@@ -105,6 +105,10 @@ int create_plan(Plan *&out, int dtype, int rank, const int64_t *extents, int axe
 int cached_plan(const Plan *&out, int dtype, int rank, const int64_t *extents, int axes, int device);
 int run_plan(const Plan &p, int direction, const void *in, void *out, double scale, void *stream);
 int current_device(int &dev);
+// hipMalloc through the library's failure-injection point (AFFT_TEST_FAIL_ALLOC), and the reset
+// of its per-thread count at the start of a plan creation
+hipError_t device_alloc(void **p, size_t bytes);
+void reset_alloc_count();
 void clear_cache();
 void *plan_alloc(Plan &owner, const DeviceInfo &dev, int64_t bytes, int &status);
 int table_pass_twiddles(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n,

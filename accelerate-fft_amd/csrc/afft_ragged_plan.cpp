@@ -244,7 +244,7 @@ static int make_exec(const RaggedPlan &rp, RaggedExec &x)
         hipEventCreateWithFlags(&x.fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&x.join, hipEventDisableTiming) != hipSuccess)
         return AFFT_ERROR_HIP;
-    if (rp.ws_bytes > 0 && hipMalloc(&x.mem, (size_t)rp.ws_bytes) != hipSuccess) return AFFT_ERROR_ALLOC;
+    if (rp.ws_bytes > 0 && device_alloc(&x.mem, (size_t)rp.ws_bytes) != hipSuccess) return AFFT_ERROR_ALLOC;
     return AFFT_SUCCESS;
 }
 
@@ -334,19 +334,21 @@ extern "C" int afft_ragged_plan_create(afft_ragged_plan **plan, int dtype, int64
     DeviceInfo dev;
     int st = device_info(device, dev);
     if (st) return st;
+    reset_alloc_count();
     int old = -1;
     (void)hipGetDevice(&old);
     if (hipSetDevice(device) != hipSuccess) return AFFT_ERROR_HIP;
     RaggedPlan *rp = new (std::nothrow) RaggedPlan();
-    if (!rp) return AFFT_ERROR_ALLOC;
+    hipStream_t s = nullptr;
+    if (!rp || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        const int err = rp ? AFFT_ERROR_HIP : AFFT_ERROR_ALLOC;
+        delete rp;
+        if (old >= 0) (void)hipSetDevice(old);
+        return err;
+    }
     rp->dtype = dtype;
     rp->device = device;
     rp->count = count;
-    hipStream_t s;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-        delete rp;
-        return AFFT_ERROR_HIP;
-    }
     st = build_ragged(*rp, lengths, dev, s);
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
