@@ -32,7 +32,6 @@ namespace afft {
 
 static const int kRadices[] = {16, 8, 5, 4, 3, 2};
 
-static thread_local int tl_row_variant = -1;
 
 // Device allocations of the library go through device_alloc.  Failure injection for tests:
 // with AFFT_TEST_FAIL_ALLOC=k set, the k-th allocation (k >= 1) of the calling thread since the
@@ -49,8 +48,6 @@ hipError_t device_alloc(void **p, size_t bytes)
     }
     return hipMalloc(p, bytes);
 }
-void force_row_variant(int v) { tl_row_variant = v; }
-int forced_row_variant() { return tl_row_variant; }
 
 // the buffer a four-step split moves its data to: output <-> work, aux <-> aux2
 static Slot other(Slot s)
@@ -226,17 +223,54 @@ struct Builder {
         return buf;
     }
 
-    // Lanes per transform of the register kernel: AFFT_REGCOL_Q (1, 2 or 4) for measurements,
-    // else the default.
-    static int regcol_lanes(int dtype, int64_t n)
+    // Lanes per transform of the register kernel: AFFT_REGCOL_Q (1, 2 or 4), else the tuned or
+    // forced choice, else one.
+    int regcol_lanes(int dtype, int64_t n)
     {
-        (void)dtype;
-        (void)n;
         if (const char *v = std::getenv("AFFT_REGCOL_Q")) {
             const int q = std::atoi(v);
             if (q == 1 || q == 2 || q == 4) return q;
         }
-        return 1;
+        const int q = tune_choice(kTuneRegcol, dtype, n, plan.device, {1, 2, 4});
+        return q > 0 ? q : 1;
+    }
+
+    // A single-pass strided tile of length n: the tuned or forced (columns, elements per thread),
+    // encoded C * 1000 + E, among the layouts of each elements-per-thread cap and the default,
+    // half and double column counts that fit the workgroup and the LDS.  Returns false to keep
+    // the default layout.
+    bool tuned_strided(int64_t n, int64_t ntrans, LdsLayout &L, int &C, int &block, size_t &lds)
+    {
+        std::vector<int> cands;
+        auto add = [&](const LdsLayout &l, int c) {
+            const int T = (int)(n / l.E);
+            if (c < 1 || (int64_t)c > std::max<int64_t>(1, ntrans)) return;
+            if ((int64_t)c * T > max_block_threads(plan.dtype == AFFT_C128, l.E)) return;
+            if (l.npass > 1 && (size_t)c * lds_padded_len((int)n) * esz() > dev.lds_bytes) return;
+            const int v = c * 1000 + l.E;
+            if (std::find(cands.begin(), cands.end(), v) == cands.end()) cands.push_back(v);
+        };
+        int c0, b0;
+        size_t l0;
+        shape(n, L.E, true, L.npass, ntrans, c0, b0, l0);
+        add(L, c0); // the default first
+        for (int maxE : {4, 8, 16, 32}) {
+            LdsLayout l{};
+            if (!choose_layout(n, plan.dtype, l, maxE)) continue;
+            int c, b;
+            size_t ld;
+            shape(n, l.E, true, l.npass, ntrans, c, b, ld);
+            for (int cc : {c, c / 2, c * 2}) add(l, cc);
+        }
+        const int v = tune_choice(kTuneStrided, plan.dtype, n, plan.device, cands);
+        if (v < 0 || v == cands.front()) return false;
+        LdsLayout l{};
+        if (!choose_layout(n, plan.dtype, l, v % 1000) || l.E != v % 1000) return false;
+        L = l;
+        C = v / 1000;
+        block = C * (int)(n / l.E);
+        lds = l.npass > 1 ? (size_t)C * lds_padded_len((int)n) * esz() : 0;
+        return true;
     }
 
     // Columns a strided workgroup aims for: neighbouring lanes take neighbouring columns, so C
@@ -334,6 +368,9 @@ struct Builder {
             status = AFFT_ERROR_UNSUPPORTED_SIZE;
             return;
         }
+        int tC = 0, tblock = 0;
+        size_t tlds = 0;
+        const bool tuned = strided && tuned_strided(n, ntrans, L, tC, tblock, tlds);
         PassTable pt{};
         const void *tw = twiddles(n, L, pt);
         if (status) return;
@@ -348,7 +385,13 @@ struct Builder {
         a.aout = aout;
         a.pt = pt;
         a.E = L.E;
-        shape(n, L.E, strided, L.npass, ntrans, a.C, l.block, l.lds);
+        if (tuned) {
+            a.C = tC;
+            l.block = tblock;
+            l.lds = tlds;
+        } else {
+            shape(n, L.E, strided, L.npass, ntrans, a.C, l.block, l.lds);
+        }
         a.lane_col = strided ? 1 : 0;
         a.lds_interleave = strided ? 1 : 0;
         a.xcd_remap = strided && !getenv_flag("AFFT_NO_XCD") ? 1 : 0;
@@ -391,7 +434,11 @@ struct Builder {
         check(gen_bluestein_kernel_f64(b64, n, M, s));
         Plan sub;
         int64_t ext[1] = {M};
+        // the filter's FP64 transform runs once, at planning time: its layout is not tuned
+        std::vector<TunePoint> *rec = tune_recording();
+        tune_record(nullptr);
         int st = build_plan(sub, AFFT_C128, 1, ext, AFFT_AXES_INNERMOST, plan.device, dev, s);
+        tune_record(rec);
         if (st == AFFT_SUCCESS) st = exec_plan(sub, -1, b64, b64, 1.0, s);
         if (st != AFFT_SUCCESS && status == AFFT_SUCCESS) status = st;
         check(convert_scale_from_f64(plan.dtype, fbuf, b64, M, 1.0 / (double)M, s));
@@ -523,16 +570,20 @@ struct Builder {
         plan.launches.push_back(l);
     }
 
-    // the variant being timed by the autotuner, else an explicit AFFT_ROW_VARIANT, else the
-    // autotuned choice, else the default layout
+    // an explicit AFFT_ROW_VARIANT, else the variant forced by the autotuner or tuned, else the
+    // default layout
     int pick_row_variant(int64_t n)
     {
-        int variant = forced_row_variant();
-        if (variant < 0) {
-            if (const char *v = std::getenv("AFFT_ROW_VARIANT")) variant = std::atoi(v);
-            else variant = tuned_row_variant(plan.dtype, n, plan.device);
+        const int nv = row_variants(plan.dtype, (int)n);
+        int variant = -1;
+        if (const char *v = std::getenv("AFFT_ROW_VARIANT")) {
+            variant = std::atoi(v);
+        } else {
+            std::vector<int> cands;
+            for (int v = 0; v < nv; ++v) cands.push_back(v);
+            variant = tune_choice(kTuneRows, plan.dtype, n, plan.device, cands);
         }
-        if (variant < 0 || variant >= row_variants(plan.dtype, (int)n)) variant = 0;
+        if (variant < 0 || variant >= nv) variant = 0;
         return variant;
     }
 

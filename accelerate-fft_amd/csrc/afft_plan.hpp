@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <utility>
@@ -118,10 +119,23 @@ int table_power(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t L, in
 int table_bluestein(Plan &owner, const DeviceInfo &dev, hipStream_t s, int64_t n, int64_t M,
                     const void *&chirp, const void *&filt);
 bool is_5smooth(int64_t n);
-int tuned_row_variant(int dtype, int64_t n, int device);
-// Row-kernel variant forced for plans built by the calling thread (-1 = none): the autotuner's
-// hook for timing each candidate.
-void force_row_variant(int v);
-int forced_row_variant();
+// The autotuner (afft_tune.cpp).  The planner asks tune_choice for every layout decision it
+// makes for a length, with the candidates it accepts (the first is its default); the answer is a
+// choice forced by the autotuner on this thread, else the tuned one from the table, else -1.
+enum TuneKind { kTuneRows = 0, kTuneStrided = 1, kTuneRegcol = 2, kTuneRagged = 3, kTuneKinds = 4 };
+struct TunePoint {
+    int kind, dtype;
+    int64_t n;
+    std::vector<int> cands;
+};
+int tune_choice(int kind, int dtype, int64_t n, int device, const std::vector<int> &cands);
+// Record the decision points the planner meets on this thread into `points` (null: stop).
+void tune_record(std::vector<TunePoint> *points);
+std::vector<TunePoint> *tune_recording();
+// Time each candidate of each point in turn, the choices made so far forced, and keep the fastest
+// in the table; time_with_forced builds and times the work, returning microseconds (< 0: failed).
+int tune_points(const std::vector<TunePoint> &points, int device, const std::function<double()> &time_with_forced);
+// Device time of one call (microseconds, < 0: failed): the median of three batches of five.
+double tune_time_calls(const std::function<int(hipStream_t)> &call, hipStream_t s);
 
 } // namespace afft

@@ -90,11 +90,36 @@ constexpr size_t kLdsBudget[2] = {40 * 1024, 72 * 1024};
 
 size_t esz_of(int dtype) { return dtype == AFFT_C64 ? 8 : 16; }
 
-// direct layout of length n within the persistent kernel's limits, or false
-bool ragged_layout(int64_t n, int dtype, LdsLayout &L)
+// direct layout of length n within the persistent kernel's limits (elements per thread at most
+// maxE), or false
+bool ragged_layout(int64_t n, int dtype, LdsLayout &L, int maxE)
 {
-    if (!choose_layout(n, dtype, L, kMaxE[dtype], kRaggedBlock)) return false;
+    if (!choose_layout(n, dtype, L, maxE, kRaggedBlock)) return false;
     return L.npass <= 1 || (size_t)lds_padded_len((int)n) * esz_of(dtype) <= kLdsBudget[dtype];
+}
+
+// The layout of a class of the persistent kernel: the default cap on elements per thread, or the
+// tuned (or forced) one among the caps that give a different valid layout.  `L` is the class's
+// true length (the autotuner's key), `len` the transform length (L, or M for Bluestein).
+bool class_layout(int64_t L, int64_t len, int dtype, int device, LdsLayout &lay)
+{
+    if (!ragged_layout(len, dtype, lay, kMaxE[dtype])) return false;
+    std::vector<int> cands = {kMaxE[dtype]};
+    std::vector<int> seen = {lay.E};
+    for (int m : {8, 4, 2}) {
+        LdsLayout l{};
+        if (m < kMaxE[dtype] && ragged_layout(len, dtype, l, m) &&
+            std::find(seen.begin(), seen.end(), l.E) == seen.end()) {
+            cands.push_back(m);
+            seen.push_back(l.E);
+        }
+    }
+    const int m = tune_choice(kTuneRagged, dtype, L, device, cands);
+    if (m > 0 && m != kMaxE[dtype]) {
+        LdsLayout l{};
+        if (ragged_layout(len, dtype, l, m)) lay = l;
+    }
+    return true;
 }
 
 // transforms per item for a direct/Bluestein class: fill the workgroup, keep LDS in budget
@@ -124,11 +149,11 @@ struct Builder {
         const void *tw = nullptr;
         int64_t len = L;
         if (is_5smooth(L)) {
-            if (!ragged_layout(L, rp.dtype, lay)) return false;
+            if (!class_layout(L, L, rp.dtype, rp.device, lay)) return false;
             K.kind = kDirect;
         } else {
             const int64_t M = bluestein_length(L);
-            if (!ragged_layout(M, rp.dtype, lay)) return false;
+            if (!class_layout(L, M, rp.dtype, rp.device, lay)) return false;
             K.kind = kBlue;
             K.M = (int)M;
             len = M;
@@ -461,6 +486,55 @@ extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *plan)
     destroy_ragged(rp);
     if (old >= 0) (void)hipSetDevice(old);
     return AFFT_SUCCESS;
+}
+
+// Tune every length class of a ragged batch on its own: the class's transforms as a ragged plan
+// of one length, its decision points (the persistent kernel's layout, or the layouts inside the
+// batched plan of a long class) timed candidate by candidate.
+extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *lengths, int device)
+{
+    if (dtype != AFFT_C64 && dtype != AFFT_C128) return AFFT_ERROR_UNSUPPORTED_DTYPE;
+    if (count < 0 || (count > 0 && !lengths) || count > (int64_t)1 << 30) return AFFT_ERROR_INVALID_VALUE;
+    std::map<int64_t, int64_t> by_len;
+    for (int64_t i = 0; i < count; ++i) {
+        if (lengths[i] < 0 || lengths[i] > ((int64_t)1 << 30)) return AFFT_ERROR_INVALID_VALUE;
+        if (lengths[i] > 0) ++by_len[lengths[i]];
+    }
+    int old = -1;
+    (void)hipGetDevice(&old);
+    if (hipSetDevice(device) != hipSuccess) return AFFT_ERROR_HIP;
+    const size_t esz = dtype == AFFT_C64 ? 8 : 16;
+    int st = AFFT_SUCCESS;
+    for (auto &kv : by_len) {
+        const std::vector<int64_t> one((size_t)kv.second, kv.first);
+        std::vector<TunePoint> points;
+        afft_ragged_plan *p = nullptr;
+        tune_record(&points);
+        st = afft_ragged_plan_create(&p, dtype, (int64_t)one.size(), one.data(), device);
+        tune_record(nullptr);
+        if (st) break;
+        (void)afft_ragged_plan_destroy(p);
+        if (points.empty()) continue;
+        void *buf = nullptr;
+        const size_t bytes = (size_t)(kv.first * kv.second) * esz;
+        if (hipMalloc(&buf, bytes) != hipSuccess || hipMemset(buf, 0, bytes) != hipSuccess) {
+            if (buf) (void)hipFree(buf);
+            st = AFFT_ERROR_ALLOC;
+            break;
+        }
+        st = tune_points(points, device, [&]() -> double {
+            afft_ragged_plan *q = nullptr;
+            if (afft_ragged_plan_create(&q, dtype, (int64_t)one.size(), one.data(), device)) return -1;
+            const double us = tune_time_calls(
+                [&](hipStream_t s) { return afft_ragged_exec(q, AFFT_FORWARD, buf, buf, s); }, nullptr);
+            (void)afft_ragged_plan_destroy(q);
+            return us;
+        });
+        (void)hipFree(buf);
+        if (st) break;
+    }
+    if (old >= 0) (void)hipSetDevice(old);
+    return st;
 }
 
 extern "C" int afft_ragged_total(const afft_ragged_plan *plan, int64_t *total)

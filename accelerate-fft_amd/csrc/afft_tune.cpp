@@ -1,23 +1,36 @@
-// afft_tune.cpp -- the autotuner and its persisted table (afft_autotune, afft_tune_save,
-// afft_tune_load in include/afft.h).
+// afft_tune.cpp -- the autotuner and its persisted table (afft_autotune, afft_ragged_autotune,
+// afft_tune_save, afft_tune_load in include/afft.h).
 //
-// What is tuned: the kernel variant of the persistent row path (elements per thread, prefetch
-// on/off, occupancy target; see afft_rows.hip) for a (dtype, length).  afft_autotune builds
-// the plan once per candidate, times each on the device with HIP events, and records the
-// fastest; plans built afterwards use it.  The table is keyed by (gfx arch, HIP runtime
-// version, dtype, length) and persisted as text, one entry per line:
+// What is tuned: every layout decision the planner makes for a length, which it asks for through
+// tune_choice():
+//   rows     the kernel variant of the persistent row path (elements per thread, prefetch,
+//            occupancy target; see afft_rows.hip)
+//   strided  a single-pass strided tile: elements per thread E (which fixes the radices and their
+//            order) and columns per workgroup C, encoded C * 1000 + E
+//   regcol   lanes per column of the register-resident kernel (1, or 2 / 4 through DPP)
+//   ragged   the cap on elements per thread of a class of the ragged persistent kernel (which
+//            fixes its radices and the transforms per work item)
+// afft_autotune records the decision points of the plan it would build, then, one point after the
+// other, builds the whole plan with each candidate forced (the choices made so far kept), times
+// it on the device with HIP events, and records the fastest; afft_ragged_autotune does the same
+// per length class of a ragged batch, timing each class alone.  Plans built afterwards use the
+// choices.  The table is keyed by (gfx arch, HIP runtime version, kind, dtype, length) and
+// persisted as text, one entry per line:
 //     afft-tune 1
-//     <arch> <hip version> rows <dtype> <length> <variant> <microseconds>
-// Lines for another arch or runtime, malformed lines and variants that no longer exist are
-// ignored on load, so a stale or corrupt file costs a re-tune, never a wrong plan.
+//     <arch> <hip version> <kind> <dtype> <length> <value> <microseconds>
+// Lines for another arch or runtime, malformed lines, unknown kinds and values the planner can
+// no longer use are ignored (on load, and again when a plan asks), so a stale or corrupt file
+// costs a re-tune, never a wrong plan.
 //
 // Environment: AFFT_AUTOTUNE=1 tunes a row length the first time a plan needs it;
 // AFFT_CACHE_DIR=<dir> loads <dir>/afft_tune.txt at the first plan and saves it after tuning.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -31,22 +44,40 @@ namespace afft {
 
 namespace {
 
+const char *const kKindName[kTuneKinds] = {"rows", "strided", "regcol", "ragged"};
+
 struct TuneEntry {
-    int variant;
+    int value;
     double us;
 };
 
+using TableKey = std::tuple<std::string, int, int, int, int64_t>; // (arch, hipver, kind, dtype, n)
 std::mutex g_mu;
-std::map<std::tuple<std::string, int, int, int64_t>, TuneEntry> g_table; // (arch, hipver, dtype, n)
+std::map<TableKey, TuneEntry> g_table;
 bool g_env_loaded = false;
+
+// choices forced on the calling thread while the autotuner times a candidate, and the decision
+// points the planner met on it while recording
+thread_local std::map<std::tuple<int, int, int64_t>, int> tl_forced;
+thread_local std::vector<TunePoint> *tl_record = nullptr;
+// set while this thread tunes a row length under g_mu (AFFT_AUTOTUNE=1): the plans it builds then
+// take only forced choices and do not look at the table
+thread_local bool tl_in_locked_tune = false;
 
 std::string arch_of(int device)
 {
+    static std::mutex mu;
+    static std::map<int, std::string> known;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(device);
+    if (it != known.end()) return it->second;
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
     std::string a = p.gcnArchName;
     const size_t colon = a.find(':');
-    return colon == std::string::npos ? a : a.substr(0, colon);
+    if (colon != std::string::npos) a = a.substr(0, colon);
+    known[device] = a;
+    return a;
 }
 
 int hip_version()
@@ -63,29 +94,51 @@ std::string cache_file()
     return std::string(d) + "/afft_tune.txt";
 }
 
+int kind_of(const char *name)
+{
+    for (int k = 0; k < kTuneKinds; ++k)
+        if (std::strcmp(name, kKindName[k]) == 0) return k;
+    return -1;
+}
+
+// values a file may hold for a kind (a plan checks again against its candidates)
+bool plausible(int kind, int dtype, int64_t n, int value)
+{
+    switch (kind) {
+    case kTuneRows: return value >= 0 && value < row_variants(dtype, (int)n);
+    case kTuneStrided: {
+        const int C = value / 1000, E = value % 1000;
+        return C >= 1 && C <= 64 && (C & (C - 1)) == 0 && E >= 1 && n % E == 0;
+    }
+    case kTuneRegcol: return (value == 1 || value == 2 || value == 4) && regcol_supported(dtype, n);
+    case kTuneRagged: return value >= 1 && value <= 32;
+    default: return false;
+    }
+}
+
 int load_locked(const char *path)
 {
     FILE *f = std::fopen(path, "r");
     if (!f) return AFFT_ERROR_IO;
     char line[512];
-    int status = AFFT_SUCCESS;
     if (!std::fgets(line, sizeof line, f) || std::strncmp(line, "afft-tune 1", 11) != 0) {
         std::fclose(f);
         return AFFT_ERROR_IO;
     }
     while (std::fgets(line, sizeof line, f)) {
-        char arch[128], kind[32];
-        int hv, dtype, variant;
+        char arch[128], kname[32];
+        int hv, dtype, value;
         long long n;
         double us;
-        if (std::sscanf(line, "%127s %d %31s %d %lld %d %lf", arch, &hv, kind, &dtype, &n, &variant, &us) != 7)
+        if (std::sscanf(line, "%127s %d %31s %d %lld %d %lf", arch, &hv, kname, &dtype, &n, &value, &us) != 7)
             continue; // malformed: ignored
-        if (std::strcmp(kind, "rows") != 0 || (dtype != AFFT_C64 && dtype != AFFT_C128) || n < 1) continue;
-        if (variant < 0 || variant >= row_variants(dtype, (int)n)) continue; // stale
-        g_table[std::make_tuple(std::string(arch), hv, dtype, (int64_t)n)] = {variant, us};
+        const int kind = kind_of(kname);
+        if (kind < 0 || (dtype != AFFT_C64 && dtype != AFFT_C128) || n < 1 || n >= ((int64_t)1 << 31)) continue;
+        if (!plausible(kind, dtype, n, value)) continue; // stale
+        g_table[TableKey(std::string(arch), hv, kind, dtype, (int64_t)n)] = {value, us};
     }
     std::fclose(f);
-    return status;
+    return AFFT_SUCCESS;
 }
 
 int save_locked(const char *path)
@@ -95,8 +148,9 @@ int save_locked(const char *path)
     if (!f) return AFFT_ERROR_IO;
     std::fprintf(f, "afft-tune 1\n");
     for (auto &kv : g_table)
-        std::fprintf(f, "%s %d rows %d %lld %d %.3f\n", std::get<0>(kv.first).c_str(), std::get<1>(kv.first),
-                     std::get<2>(kv.first), (long long)std::get<3>(kv.first), kv.second.variant, kv.second.us);
+        std::fprintf(f, "%s %d %s %d %lld %d %.3f\n", std::get<0>(kv.first).c_str(), std::get<1>(kv.first),
+                     kKindName[std::get<2>(kv.first)], std::get<3>(kv.first), (long long)std::get<4>(kv.first),
+                     kv.second.value, kv.second.us);
     const bool ok = std::fclose(f) == 0;
     if (!ok || std::rename(tmp.c_str(), path) != 0) {
         std::remove(tmp.c_str());
@@ -113,33 +167,59 @@ void ensure_env_loaded_locked()
     if (!f.empty()) (void)load_locked(f.c_str());
 }
 
-// time one row-plan variant on a (rows x n) batch; returns microseconds or a negative value
-double time_variant(int dtype, int64_t rows, int64_t n, int variant, int device)
+void save_env_locked()
 {
-    force_row_variant(variant);
-    Plan *p = nullptr;
+    const std::string f = cache_file();
+    if (!f.empty()) (void)save_locked(f.c_str());
+}
+
+} // namespace
+
+// Device time of one call in microseconds: the median of three batches of five calls after three
+// warm-up calls, or a negative value when a call fails.
+double tune_time_calls(const std::function<int(hipStream_t)> &call, hipStream_t s)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    double us = -1;
+    if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; ++i) ok = call(s) == AFFT_SUCCESS;
+        std::vector<double> t;
+        for (int rep = 0; rep < 3 && ok; ++rep) {
+            ok = hipEventRecord(a, s) == hipSuccess;
+            for (int i = 0; i < 5 && ok; ++i) ok = call(s) == AFFT_SUCCESS;
+            ok = ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
+            float ms = 0;
+            if (ok && hipEventElapsedTime(&ms, a, b) == hipSuccess) t.push_back(1e3 * ms / 5);
+        }
+        if (ok && t.size() == 3) {
+            std::sort(t.begin(), t.end());
+            us = t[1];
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return us;
+}
+
+namespace {
+
+// time one row-plan variant on a (rows x n) batch; microseconds or a negative value
+double time_row_variant(int dtype, int64_t n, int variant, int device)
+{
+    const int64_t rows = std::max<int64_t>(1, ((int64_t)256 << 20) / (n * (dtype == AFFT_C64 ? 8 : 16)));
     const int64_t ext[2] = {rows, n};
+    tl_forced[std::make_tuple((int)kTuneRows, dtype, n)] = variant;
+    Plan *p = nullptr;
     const int st = create_plan(p, dtype, 2, ext, AFFT_AXES_INNERMOST, device);
-    force_row_variant(-1);
+    tl_forced.erase(std::make_tuple((int)kTuneRows, dtype, n));
     if (st) return -1;
     const size_t bytes = (size_t)(rows * n) * (dtype == AFFT_C64 ? 8 : 16);
     void *in = nullptr, *out = nullptr;
     double us = -1;
-    hipEvent_t a = nullptr, b = nullptr;
     if (hipMalloc(&in, bytes) == hipSuccess && hipMalloc(&out, bytes) == hipSuccess &&
-        hipMemset(in, 0, bytes) == hipSuccess && hipEventCreate(&a) == hipSuccess &&
-        hipEventCreate(&b) == hipSuccess) {
-        bool ok = true;
-        for (int i = 0; i < 3 && ok; ++i) ok = exec_plan(*p, -1, in, out, 1.0, nullptr) == AFFT_SUCCESS;
-        const int reps = 10;
-        ok = ok && hipEventRecord(a, nullptr) == hipSuccess;
-        for (int i = 0; i < reps && ok; ++i) ok = exec_plan(*p, -1, in, out, 1.0, nullptr) == AFFT_SUCCESS;
-        ok = ok && hipEventRecord(b, nullptr) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
-        float ms = 0;
-        if (ok && hipEventElapsedTime(&ms, a, b) == hipSuccess) us = 1e3 * ms / reps;
-    }
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
+        hipMemset(in, 0, bytes) == hipSuccess)
+        us = tune_time_calls([&](hipStream_t s) { return exec_plan(*p, -1, in, out, 1.0, s); }, nullptr);
     if (in) (void)hipFree(in);
     if (out) (void)hipFree(out);
     free_plan(*p);
@@ -152,42 +232,83 @@ int tune_rows_locked(int dtype, int64_t n, int device, int &best)
     const int nv = row_variants(dtype, (int)n);
     best = 0;
     if (nv <= 1) return AFFT_SUCCESS;
-    // a batch of about 256 MiB keeps every variant's persistent grid full
-    const int64_t rows = std::max<int64_t>(1, ((int64_t)256 << 20) / (n * (dtype == AFFT_C64 ? 8 : 16)));
     double best_us = 1e300;
+    tl_in_locked_tune = true;
     for (int v = 0; v < nv; ++v) {
-        const double us = time_variant(dtype, rows, n, v, device);
+        const double us = time_row_variant(dtype, n, v, device);
         if (us > 0 && us < best_us) {
             best_us = us;
             best = v;
         }
     }
+    tl_in_locked_tune = false;
     if (best_us == 1e300) return AFFT_ERROR_HIP;
-    g_table[std::make_tuple(arch_of(device), hip_version(), dtype, n)] = {best, best_us};
+    g_table[TableKey(arch_of(device), hip_version(), kTuneRows, dtype, n)] = {best, best_us};
     return AFFT_SUCCESS;
 }
 
 } // namespace
 
-// The tuned row variant for (dtype, n) on `device`, tuning first when AFFT_AUTOTUNE=1; -1 when
-// there is no entry.
-int tuned_row_variant(int dtype, int64_t n, int device)
+int tune_choice(int kind, int dtype, int64_t n, int device, const std::vector<int> &cands)
 {
+    if (tl_record) {
+        const bool seen = std::any_of(tl_record->begin(), tl_record->end(), [&](const TunePoint &p) {
+            return p.kind == kind && p.dtype == dtype && p.n == n;
+        });
+        if (!seen && cands.size() > 1) tl_record->push_back(TunePoint{kind, dtype, n, cands});
+    }
+    auto f = tl_forced.find(std::make_tuple(kind, dtype, n));
+    if (f != tl_forced.end() && std::find(cands.begin(), cands.end(), f->second) != cands.end()) return f->second;
+    if (tl_in_locked_tune) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     ensure_env_loaded_locked();
-    const auto key = std::make_tuple(arch_of(device), hip_version(), dtype, n);
-    auto it = g_table.find(key);
-    if (it != g_table.end()) return it->second.variant;
-    const char *at = std::getenv("AFFT_AUTOTUNE");
-    if (at && at[0] == '1' && row_variants(dtype, (int)n) > 1) {
-        int best = 0;
-        if (tune_rows_locked(dtype, n, device, best) == AFFT_SUCCESS) {
-            const std::string f = cache_file();
-            if (!f.empty()) (void)save_locked(f.c_str());
-            return best;
+    auto it = g_table.find(TableKey(arch_of(device), hip_version(), kind, dtype, n));
+    if (it != g_table.end() && std::find(cands.begin(), cands.end(), it->second.value) != cands.end())
+        return it->second.value;
+    if (kind == kTuneRows) {
+        const char *at = std::getenv("AFFT_AUTOTUNE");
+        if (at && at[0] == '1' && row_variants(dtype, (int)n) > 1) {
+            int best = 0;
+            if (tune_rows_locked(dtype, n, device, best) == AFFT_SUCCESS) {
+                save_env_locked();
+                return best;
+            }
         }
     }
     return -1;
+}
+
+void tune_record(std::vector<TunePoint> *points) { tl_record = points; }
+std::vector<TunePoint> *tune_recording() { return tl_record; }
+
+int tune_points(const std::vector<TunePoint> &points, int device,
+                const std::function<double()> &time_with_forced)
+{
+    int st = AFFT_SUCCESS;
+    for (const TunePoint &p : points) {
+        const auto key = std::make_tuple(p.kind, p.dtype, p.n);
+        double best_us = 1e300;
+        int best = p.cands.front();
+        for (int v : p.cands) {
+            tl_forced[key] = v;
+            const double us = time_with_forced();
+            if (us > 0 && us < best_us) {
+                best_us = us;
+                best = v;
+            }
+        }
+        if (best_us == 1e300) {
+            st = AFFT_ERROR_HIP;
+            break;
+        }
+        tl_forced[key] = best; // later points are timed with this choice
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_table[TableKey(arch_of(device), hip_version(), p.kind, p.dtype, p.n)] = {best, best_us};
+    }
+    tl_forced.clear();
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!st) save_env_locked();
+    return st;
 }
 
 } // namespace afft
@@ -201,28 +322,51 @@ extern "C" int afft_autotune(int dtype, int rank, const int64_t *extents, int ax
     if (!extents) return AFFT_ERROR_INVALID_VALUE;
     if (axes != AFFT_AXES_INNERMOST && axes != AFFT_AXES_ALL && axes != AFFT_AXES_LAST2)
         return AFFT_ERROR_INVALID_VALUE;
+    int64_t total = 1;
+    for (int i = 0; i < rank; ++i) {
+        if (extents[i] < 0) return AFFT_ERROR_INVALID_VALUE;
+        if (__builtin_mul_overflow(total, extents[i], &total)) return AFFT_ERROR_UNSUPPORTED_SIZE;
+    }
+    if (total == 0) return AFFT_SUCCESS;
     DeviceInfo dev;
     int st = device_info(device, dev);
     if (st) return st;
     int old = -1;
     (void)hipGetDevice(&old);
     if (hipSetDevice(device) != hipSuccess) return AFFT_ERROR_HIP;
-    // every contiguous-row length the plan would run with the persistent row kernel
-    std::vector<int64_t> lens;
-    if (extents[rank - 1] > 1) lens.push_back(extents[rank - 1]);
+    // the decision points of the plan, recorded while it is built once
+    std::vector<TunePoint> points;
     {
-        std::lock_guard<std::mutex> lk(g_mu);
-        ensure_env_loaded_locked();
-        for (int64_t n : lens) {
-            if (row_variants(dtype, (int)n) <= 1) continue;
-            int best = 0;
-            st = tune_rows_locked(dtype, n, device, best);
-            if (st) break;
+        Plan *p = nullptr;
+        tune_record(&points);
+        st = create_plan(p, dtype, rank, extents, axes, device);
+        tune_record(nullptr);
+        if (!st) {
+            free_plan(*p);
+            delete p;
         }
-        const std::string f = cache_file();
-        if (!st && !f.empty()) (void)save_locked(f.c_str());
     }
-    // plans built from now on pick up the tuned variant
+    const size_t bytes = (size_t)total * (dtype == AFFT_C64 ? 8 : 16);
+    void *in = nullptr, *out = nullptr;
+    if (!st && !points.empty()) {
+        if (hipMalloc(&in, bytes) != hipSuccess || hipMalloc(&out, bytes) != hipSuccess ||
+            hipMemset(in, 0, bytes) != hipSuccess)
+            st = AFFT_ERROR_ALLOC;
+    }
+    if (!st && !points.empty()) {
+        st = tune_points(points, device, [&]() -> double {
+            Plan *p = nullptr;
+            if (create_plan(p, dtype, rank, extents, axes, device)) return -1;
+            const double us = tune_time_calls([&](hipStream_t s) { return exec_plan(*p, -1, in, out, 1.0, s); }, nullptr);
+            (void)hipDeviceSynchronize();
+            free_plan(*p);
+            delete p;
+            return us;
+        });
+    }
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    // plans built from now on pick up the tuned choices
     clear_cache();
     if (old >= 0) (void)hipSetDevice(old);
     return st;

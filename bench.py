@@ -279,25 +279,31 @@ def cpu_baseline(x, rows, n, budget_s=6.0):
 # Workloads beyond the headline
 
 def ragged_extras(afft, Mode, torch, dev, args, chk):
-    """10,000 Complex Float transforms of mixed lengths through one ragged plan (the lengths that fit
-    one workgroup in one persistent launch, the longer ones as batched plans over their gathered
-    rows; eager and as a replayed hipGraph) against one plain call per transform and one batched
+    """10,000 Complex Float transforms of mixed lengths through one autotuned ragged plan (the
+    lengths that fit one workgroup in one persistent launch, the longer ones as batched plans over
+    their gathered rows; eager and as a replayed hipGraph) against one plain call per transform and one batched
     call per length, the latter both on pre-gathered batches and on the packed buffer (gather,
     call, scatter)."""
     import ctypes
     lengths = ragged_lengths()
     total = sum(lengths)
+    # autotune every length class first (not timed; the choices also reach the per-length plans
+    # of the comparison, which share the tuned row layouts)
+    lib = afft._lib.lib()
+    t0 = time.perf_counter()
+    st = lib.afft_ragged_autotune(0, len(lengths), (ctypes.c_int64 * len(lengths))(*lengths), 0)
+    tune_s = time.perf_counter() - t0
     xh = splitmix_uniform(4, (total,), "complex64")
     x = to_device(xh, dev)
     out = torch.empty_like(x)
     plan = afft.RaggedPlan(lengths, torch.complex64)
     flops = sum(5.0 * n * math.log2(n) for n in lengths)
-    res = {"ragged_total_elements": total}
+    res = {"ragged_total_elements": total,
+           "ragged_autotune": {"status": int(st), "seconds": round(tune_s, 2)}}
     t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(5, args.steps // 2), 3)
     tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(5, args.steps // 2), 3)
     got = out.cpu().numpy()
     # plain: one fft1D call per transform, straight through the C ABI (no tensor allocation)
-    lib = afft._lib.lib()
     offs = [0]
     for n in lengths[:-1]:
         offs.append(offs[-1] + n)

@@ -251,10 +251,17 @@ int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w,
                                  const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
                                  void *stream);
 
-/* Autotuning: time the candidate kernel layouts for a key on the device and remember the best in
- * the plan cache; afft_tune_save/afft_tune_load persist the choices as a text file keyed by
- * (gfx arch, HIP runtime version, dtype, length, layout). */
+/* Autotuning: time the candidate layouts of every decision the planner makes for a shape (the row
+ * kernel variant, elements per thread and columns per workgroup of strided tiles -- which fix the
+ * radices and their order --, the lanes per column of the register kernel) on the device, one
+ * decision after the other, and remember the fastest; plans built afterwards use them (the plan
+ * cache is cleared).  afft_ragged_autotune does the same for every length class of a ragged batch,
+ * each timed alone (elements per thread of the persistent kernel's classes, the layouts of the
+ * long classes' plans).  afft_tune_save/afft_tune_load persist the choices as a text file keyed by
+ * (gfx arch, HIP runtime version, decision kind, dtype, length); stale or malformed lines are
+ * ignored. */
 int afft_autotune(int dtype, int rank, const int64_t *extents, int axes, int device);
+int afft_ragged_autotune(int dtype, int64_t count, const int64_t *lengths, int device);
 int afft_tune_save(const char *path);
 int afft_tune_load(const char *path);
 

@@ -125,11 +125,21 @@ def test_tune_table_file_handling(tmp_path):
                   "gfx950 70226015 rows 0 4096 1 190.0\n"
                   "gfx950 70226015 rows 0 4096 999 1.0\n"
                   "garbage line\n"
-                  "gfx950 70226015 rows 7 4096 0 1.0\n")
+                  "gfx950 70226015 rows 7 4096 0 1.0\n"
+                  "gfx950 70226015 strided 0 512 16016 44.0\n"
+                  "gfx950 70226015 strided 0 512 16007 44.0\n"
+                  "gfx950 70226015 regcol 0 64 4 40.0\n"
+                  "gfx950 70226015 regcol 0 64 3 40.0\n"
+                  "gfx950 70226015 ragged 0 1000 8 5.0\n"
+                  "gfx950 70226015 ragged 0 1000 999 5.0\n"
+                  "gfx950 70226015 nonsense 0 1000 8 5.0\n")
     assert lib.afft_tune_load(str(ok).encode()) == 0
     out = tmp_path / "out.txt"
     assert lib.afft_tune_save(str(out).encode()) == 0
     lines = out.read_text().splitlines()
     assert lines[0] == "afft-tune 1"
     assert any(l.split()[:6] == ["gfx950", "70226015", "rows", "0", "4096", "1"] for l in lines[1:])
-    assert not any(" 999 " in l for l in lines)
+    assert any(l.split()[2:6] == ["strided", "0", "512", "16016"] for l in lines[1:])
+    assert any(l.split()[2:6] == ["regcol", "0", "64", "4"] for l in lines[1:])
+    assert any(l.split()[2:6] == ["ragged", "0", "1000", "8"] for l in lines[1:])
+    assert not any(" 999 " in l or " 16007 " in l or "regcol 0 64 3 " in l or "nonsense" in l for l in lines)

@@ -39,15 +39,53 @@ def test_every_row_variant(gpu, dtype, n, monkeypatch):
 
 def test_autotune_round_trip(gpu, tmp_path):
     lib = afft._lib.lib()
-    ext = afft._lib.extents_arg([64, 4096])
-    assert lib.afft_autotune(0, 2, ext, 0, 0) == 0
+    # a row length, a strided tile (the 512-point columns of a batched 2D) and a register column
+    # pass (the split axis of fft2D 4096^2): every decision point of each plan is timed
+    assert lib.afft_autotune(0, 2, afft._lib.extents_arg([64, 4096]), 0, 0) == 0
+    assert lib.afft_autotune(0, 3, afft._lib.extents_arg([16, 512, 512]), 2, 0) == 0
+    assert lib.afft_autotune(0, 2, afft._lib.extents_arg([1024, 4096]), 1, 0) == 0
+    # a ragged batch: every class timed alone (persistent-kernel classes and a long class)
+    lengths = [64] * 300 + [1000] * 60 + [4096] * 30 + [257] * 40 + [20480] * 8
+    arr = (ctypes.c_int64 * len(lengths))(*lengths)
+    assert lib.afft_ragged_autotune(0, len(lengths), arr, 0) == 0
     path = tmp_path / "tune.txt"
     assert lib.afft_tune_save(str(path).encode()) == 0
     lines = path.read_text().splitlines()
     assert lines[0] == "afft-tune 1"
-    entry = [l.split() for l in lines[1:] if l.split()[2:5] == ["rows", "0", "4096"]]
-    assert entry and entry[0][0].startswith("gfx")
+    rows = [l.split() for l in lines[1:]]
+    kinds = {r[2] for r in rows}
+    assert {"rows", "strided", "regcol", "ragged"} <= kinds, kinds
+    assert any(r[2:5] == ["rows", "0", "4096"] for r in rows)
+    assert any(r[2:5] == ["strided", "0", "512"] for r in rows)
+    assert any(r[2:5] == ["regcol", "0", "64"] for r in rows)
+    assert any(r[2] == "ragged" and r[4] in ("64", "1000", "4096", "257") for r in rows)
+    assert all(r[0].startswith("gfx") for r in rows)
+    # a stale and a corrupt entry next to the good ones are ignored on load
+    arch, hv = rows[0][0], rows[0][1]
+    text = path.read_text() + f"{arch} {hv} strided 0 512 16007 1.0\n{arch} {hv} ragged 0 64 999 1.0\n" \
+                              f"{arch} {hv} regcol 0 64 3 1.0\nnot an entry\n"
+    path.write_text(text)
     assert lib.afft_tune_load(str(path).encode()) == 0
-    x = torch.from_numpy(O.uniform(3, (64, 4096))).to(torch.complex64).to(gpu)
-    y = afft.fft(Mode.Forward, x)
-    assert O.rel_l2(y.cpu().numpy(), O.fft(O.Mode.Forward, x.cpu().numpy())) <= tolerance(False, 4096)
+    again = tmp_path / "again.txt"
+    assert lib.afft_tune_save(str(again).encode()) == 0
+    kept = again.read_text()
+    assert " 16007 " not in kept and " 999 " not in kept and "regcol 0 64 3 " not in kept
+    # plans built with the tuned choices are still right
+    x = torch.from_numpy(O.uniform(3, (16, 512, 512))).to(torch.complex64).to(gpu)
+    y = afft.fft2D_many(Mode.Forward, x)
+    assert O.rel_l2(y.cpu().numpy(), O.fft_axes(x.cpu().numpy(), (1, 2), -1)) <= tolerance(False, 512 * 512)
+    z = torch.from_numpy(O.uniform(4, (1024, 4096))).to(torch.complex64).to(gpu)
+    w = afft.fft2D(Mode.Inverse, z)
+    assert O.rel_l2(w.cpu().numpy(), O.fft2D(O.Mode.Inverse, z.cpu().numpy())) <= tolerance(False, 1024 * 4096)
+    total = sum(lengths)
+    r = torch.from_numpy(O.uniform(5, (total,))).to(torch.complex64).to(gpu)
+    plan = afft.RaggedPlan(lengths, torch.complex64)
+    out = plan.execute(Mode.Forward, r).cpu().numpy()
+    xr = r.cpu().numpy()
+    off = 0
+    for n in (64, 1000, 4096, 257, 20480):
+        cnt = lengths.count(n)
+        got = out[off:off + cnt * n].reshape(cnt, n)
+        assert O.rel_l2(got, O.fft(O.Mode.Forward, xr[off:off + cnt * n].reshape(cnt, n))) <= tolerance(False, n), n
+        off += cnt * n
+    plan.close()
