@@ -44,20 +44,30 @@ template <int T> __device__ __forceinline__ void row_sync()
     else lds_sync();
 }
 
-template <int N, int E> struct RowShape {
+// T threads per row, C rows per tile (a workgroup of about 256 threads), at least CM rows: the
+// transposed-store form needs 128-byte runs of neighbouring rows (16 in FP32), or the runs of
+// neighbouring tiles share lines and the partial lines cost extra write traffic (measured on the
+// 2^22 split six-step with 8-row tiles: 1.48x the array written)
+template <int N, int E, int CM = 1> struct RowShape {
     static constexpr int T = N / E;
-    static constexpr int C = T >= 256 ? 1 : 256 / T;
+    static constexpr int C0 = T >= 256 ? 1 : 256 / T;
+    static constexpr int C = C0 >= CM || CM * T > 1024 ? C0 : CM;
     static constexpr int BLOCK = C * T;
 };
 
+// minimum rows per tile of the transposed-store form: one 128-byte line of elements
+template <typename Real, bool TR> constexpr int row_min_tile() { return TR ? 64 / (int)sizeof(Real) : 1; }
+
 template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, int... Rs>
-__global__ void __launch_bounds__((RowShape<N, E>::BLOCK)) __attribute__((amdgpu_waves_per_eu(W, 8)))
+__global__ void __launch_bounds__((RowShape<N, E, row_min_tile<Real, TR>()>::BLOCK))
+__attribute__((amdgpu_waves_per_eu(W, 8)))
 afft_rows(const RowArgs a)
 {
     using V = typename cplx_of<Real>::type;
     using RL = Radices<Rs...>;
-    constexpr int T = RowShape<N, E>::T;
-    constexpr int C = RowShape<N, E>::C;
+    using Shape = RowShape<N, E, row_min_tile<Real, TR>()>;
+    constexpr int T = Shape::T;
+    constexpr int C = Shape::C;
     constexpr int NP = RL::NP;
     constexpr int PADN = lds_padded_len(N);
     static_assert(T * E == N, "E must divide N");
@@ -246,13 +256,16 @@ struct RowEntry {
     RowFn fn;
     int block;
     bool tr;
+    int rows; // rows per tile
 };
 
 #define AFFT_ROW(REAL, DT, N, E, PF, W, LD, ...)                                                 \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, false}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, false, \
+             RowShape<N, E>::C}
 // the transposed-store form, one per length (selected with variant kRowTransposed)
 #define AFFT_ROW_TR(REAL, DT, N, E, PF, W, LD, ...)                                              \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, __VA_ARGS__>, RowShape<N, E>::BLOCK, true}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, __VA_ARGS__>,                  \
+             RowShape<N, E, row_min_tile<REAL, true>()>::BLOCK, true, RowShape<N, E, row_min_tile<REAL, true>()>::C}
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
@@ -333,8 +346,7 @@ size_t row_lds_bytes(int dtype, int n, int variant)
 {
     const RowEntry *e = find_row(dtype, n, variant);
     if (!e) return 0;
-    const int T = e->n / e->E;
-    const int C = T >= 256 ? 1 : 256 / T;
+    const int C = e->rows;
     size_t elems = (size_t)C * lds_padded_len(e->n);
     if (e->tr) elems = std::max(elems, (size_t)e->n * (size_t)(C + 1)); // the regroup buffer
     return elems * (dtype == 0 ? 8 : 16);
@@ -343,9 +355,7 @@ size_t row_lds_bytes(int dtype, int n, int variant)
 int row_rows_per_tile(int dtype, int n, int variant)
 {
     const RowEntry *e = find_row(dtype, n, variant);
-    if (!e) return 0;
-    const int T = e->n / e->E;
-    return T >= 256 ? 1 : 256 / T;
+    return e ? e->rows : 0;
 }
 
 int row_occupancy(int dtype, int n, int variant)
