@@ -170,6 +170,10 @@ struct RowArgs {
     // element k2 stored at (tr_toff ? tr_toff[o] : o * N * n1) + k1 + n1 * k2
     int64_t tr_n1;
     const int64_t *tr_toff;
+    // rotated tiles (tr_rot, needs C | n1): tile slot u of transform o holds row
+    // k1 = (u - s_o) mod n1, s_o = the element address of X_o[0] modulo C, so that each tile's
+    // stores fill aligned runs of C elements wherever the transform starts
+    int tr_rot;
 };
 
 // afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,

@@ -686,15 +686,16 @@ struct Builder {
         add_stockham(n1, o * n2, ia, ia, true, src, cur, n, 1, hi, lo, sh);
         if (status) return;
         // rows short enough for a tile of several whole rows store transposed themselves: two
-        // passes instead of three.  Their stores are runs of 8-32 neighbouring k1 (64-256 bytes),
-        // which need the output aligned: rows gathered at arbitrary offsets keep the transpose
-        // pass, whose runs are 512 bytes (measured in a ragged batch: 40960 x 295 took 111 us
-        // with transposed rows against 80 us for rows plus transpose)
+        // passes instead of three.  A tile's stores are runs of C neighbouring k1 (one 128-byte
+        // line or more); its rows are rotated by the output's alignment (RowArgs::tr_rot), so the
+        // runs stay whole lines also for transforms that start at arbitrary offsets, as the rows
+        // of a ragged batch do (unrotated, their runs straddle two lines shared with the next
+        // tile: 40960 x 295 took 111 us with transposed rows against 80 us for rows plus transpose)
         const int tr_rows = row_rows_per_tile(plan.dtype, (int)n2, kRowTransposed);
-        if (tr_rows > 1 && n1 % tr_rows == 0 && !(plan.flags & kPlanUnalignedRows) &&
-            !getenv_flag("AFFT_NO_ROWS") && !getenv_flag("AFFT_NO_ROW_TR")) {
+        if (tr_rows > 1 && n1 % tr_rows == 0 && !getenv_flag("AFFT_NO_ROWS") && !getenv_flag("AFFT_NO_ROW_TR")) {
             add_rows(n2, o * n1, cur, next, kRowTransposed);
             plan.launches.back().ra.tr_n1 = n1;
+            plan.launches.back().ra.tr_rot = 1;
             return;
         }
         if (row_variants(plan.dtype, (int)n2) > 0 && !getenv_flag("AFFT_NO_ROWS")) {

@@ -87,10 +87,37 @@ afft_rows(const RowArgs a)
 
     const bool nt_in = a.nt_in, nt_out = a.nt_out;
     const int64_t *__restrict__ roffs = a.roffs;
+    // transposed stores of transforms that start anywhere (tr_rot): tile slot r of transform o
+    // holds row k1 = (u - s) mod n1, u = r - o n1, s = address of X_o[0] in elements modulo C.
+    // The tile's C rows then store runs base - s + jC + [0, C) at every k2 (the rows past
+    // n1 - s in the run of k2 + 1), which are whole aligned lines written by this workgroup
+    // alone, where unrotated tiles would write runs straddling the lines of the next tile.
+    auto tr_row = [&](int64_t row, int64_t &base, int64_t &k1) -> int64_t {
+        const int64_t n1 = a.tr_n1;
+        const int64_t o = row / n1;
+        const int64_t u = row - o * n1;
+        base = a.tr_toff ? a.tr_toff[o] : o * (int64_t)N * n1;
+        k1 = u;
+        if (a.tr_rot) {
+            const int64_t s = (int64_t)((reinterpret_cast<uintptr_t>(out) / sizeof(V) + (uint64_t)base) % C);
+            k1 = u >= s ? u - s : u - s + n1;
+        }
+        return o * n1 + k1;
+    };
+    // the row a tile slot reads: itself, or its rotated row
+    auto src_row = [&](int64_t row) -> int64_t {
+        if constexpr (TR) {
+            if (a.tr_rot) {
+                int64_t base, k1;
+                return tr_row(row, base, k1);
+            }
+        }
+        return row;
+    };
     V nxt[E];
     auto load_tile = [&](int64_t tile) {
-        const int64_t row = tile * C + c;
-        if (tile < ntiles && row < nrows) {
+        if (tile < ntiles && tile * C + c < nrows) {
+            const int64_t row = src_row(tile * C + c);
             const V *p = in + (roffs ? roffs[row] : row * N) + t;
             if (nt_in) {
 #pragma unroll
@@ -142,7 +169,8 @@ afft_rows(const RowArgs a)
 #pragma unroll
         for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
         if (a.sg_in.on && gt * C + c < nrows) {
-            const int64_t rb = roffs ? roffs[gt * C + c] : (gt * C + c) * N;
+            const int64_t r0 = src_row(gt * C + c);
+            const int64_t rb = roffs ? roffs[r0] : r0 * N;
 #pragma unroll
             for (int i = 0; i < E; ++i) reg[i] = sign_apply(a.sg_in, rb + t + i * T, reg[i]);
         }
@@ -207,8 +235,9 @@ afft_rows(const RowArgs a)
             const int64_t row2 = gt * C + c2;
             if (row2 < nrows) {
                 const int64_t n1 = a.tr_n1;
-                const int64_t o = row2 / n1, k1 = row2 - o * n1;
-                const int64_t ob = (a.tr_toff ? a.tr_toff[o] : o * (int64_t)N * n1) + k1;
+                int64_t base, k1;
+                (void)tr_row(row2, base, k1);
+                const int64_t ob = base + k1;
 #pragma unroll
                 for (int i = 0; i < E; ++i) {
                     const int64_t k2 = t2 + i * T;

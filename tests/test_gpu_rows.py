@@ -89,3 +89,25 @@ def test_autotune_round_trip(gpu, tmp_path):
         assert O.rel_l2(got, O.fft(O.Mode.Forward, xr[off:off + cnt * n].reshape(cnt, n))) <= tolerance(False, n), n
         off += cnt * n
     plan.close()
+
+
+@pytest.mark.parametrize("n", [20480, 24576, 40960])
+def test_transposed_rows_at_every_output_alignment(gpu, n):
+    # a six-step plan 64 x (n / 64) whose last pass stores transposed, in tiles of rows rotated by
+    # the output's alignment.  The rotation only moves rows between workgroups, so an output at
+    # any element offset holds the aligned result bit for bit, and nothing around it is written.
+    shape = (3, n)
+    plan = afft.Plan(shape, torch.complex64, "innermost")
+    xh = O.uniform(31, shape)
+    x = torch.from_numpy(xh).to(torch.complex64).to(gpu)
+    ref = plan.execute(x, -1, 1.0)
+    assert O.rel_l2(ref.cpu().numpy(), O.fft(O.Mode.Forward, x.cpu().numpy())) <= tolerance(False, n)
+    total = x.numel()
+    buf = torch.empty(total + 40, dtype=torch.complex64, device=gpu)
+    for off in (1, 3, 8, 15, 16, 17, 29):
+        buf.fill_(complex(float("nan"), float("nan")))
+        out = buf[off:off + total].view(shape)
+        plan.execute(x, -1, 1.0, out=out)
+        assert torch.equal(out, ref), off
+        assert torch.isnan(buf[:off]).all() and torch.isnan(buf[off + total:]).all(), off
+    plan.close()
