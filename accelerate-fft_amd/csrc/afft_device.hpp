@@ -597,9 +597,15 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
     }
 
     // PAD: positions pass through lds_pad (a private contiguous slot, es = 1); interleaved tiles
-    // (es = C) measured the same without it
-    template <bool PAD = false> __device__ __forceinline__ void run(V (&reg)[E], V *base, int es, int t) const
+    // (es = C) measured the same without it.  WAVE: the T threads of the transform sit in one
+    // wavefront with a private slot, so the exchanges order at wavefront scope (no barrier).
+    // FRESH: each exchange recomputes its LDS positions from the thread index (passed through an
+    // empty asm), instead of the compiler computing every pass's positions up front and keeping
+    // them live across the transform.
+    template <bool PAD = false, bool WAVE = false, bool FRESH = false>
+    __device__ __forceinline__ void run(V (&reg)[E], V *base, int es, int t0) const
     {
+        static_assert(!WAVE || (T <= 64 && 64 % T == 0), "wavefront-private exchanges");
         static_for<NP>([&](auto pc) {
             constexpr int p = decltype(pc)::value;
             constexpr int R = RL::R[p];
@@ -626,7 +632,11 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
                 for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
             });
             if constexpr (p + 1 < NP) {
-                lds_sync(); // the previous exchange has been read by every thread
+                int t = t0;
+                if constexpr (FRESH) __asm__ volatile("" : "+v"(t));
+                // the previous exchange has been read by every thread
+                if constexpr (WAVE) wave_sync();
+                else lds_sync();
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     const int b = t + j * T;
@@ -635,7 +645,8 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
 #pragma unroll
                     for (int r = 0; r < R; ++r) base[pos<PAD>(dst + r * Ns) * es] = reg[j + r * B];
                 }
-                lds_sync();
+                if constexpr (WAVE) wave_sync();
+                else lds_sync();
 #pragma unroll
                 for (int i = 0; i < E; ++i) reg[i] = base[pos<PAD>(t + i * T) * es];
             }

@@ -383,10 +383,12 @@ static hipError_t launch_bluestein_t(const BluesteinArgs &a, int block, int64_t 
 {
     if constexpr (sizeof(Real) == 4) {
         // compile-time transforms for the padded lengths of two common primes (257 -> 640,
-        // 4099 -> 10240), with the radices the layout chooser picks
+        // 4099 -> 10240): the radices the layout chooser picks, and for 10240 also the
+        // 512-thread layout of contiguous rows
         if (a.pw) {
             if (a.pt.n == 640 && a.E == 40) return launch_one(afft_bluestein<Real, 40, 640, 8, 8, 5, 2>, a, block, grid, lds, s);
             if (a.pt.n == 10240 && a.E == 40) return launch_one(afft_bluestein<Real, 40, 10240, 8, 8, 8, 5, 4>, a, block, grid, lds, s);
+            if (a.pt.n == 10240 && a.E == 20) return launch_one(afft_bluestein<Real, 20, 10240, 2, 4, 4, 4, 4, 4, 5>, a, block, grid, lds, s);
         }
     }
     switch (a.E) {

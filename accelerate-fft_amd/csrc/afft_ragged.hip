@@ -100,6 +100,67 @@ __device__ __forceinline__ void run_blue(const RaggedArgs &a, const RaggedClass 
     }
 }
 
+// A batch of C = kRaggedBlock / T transforms of a compiled layout (direct, or the two transforms
+// of a chirp-z class of padded length N): the passes of CtStockham, every radix, stride and
+// twiddle index a compile-time constant.  Transforms of up to 64 threads exchange within their
+// wavefront.
+template <typename Real, int N, int E, int... Rs>
+__device__ __noinline__ void run_ct(const RaggedArgs &a, const RaggedClass &K, const RaggedItem &it, void *lds_raw)
+{
+    using V = typename cplx_of<Real>::type;
+    using CT = CtStockham<V, N, E, Rs...>;
+    constexpr int T = N / E;
+    constexpr bool kWave = T <= 64 && 64 % T == 0;
+    static_assert(kRaggedBlock % T == 0, "whole transforms per workgroup");
+    V *lds = reinterpret_cast<V *>(lds_raw);
+    const int t = threadIdx.x % T, c = threadIdx.x / T;
+    const bool active = c < it.count;
+    const bool blue = K.kind == kBlue;
+    const int n = K.L; // the transform's own length: N, or the chirp-z length below N
+    const int64_t off = active ? a.moffs[it.first + c] : 0;
+    const V *in = reinterpret_cast<const V *>(a.in);
+    V *out = reinterpret_cast<V *>(a.out);
+    const V *chirp = reinterpret_cast<const V *>(K.chirp);
+    CT ct;
+    ct.load(reinterpret_cast<const V *>(K.tw), t);
+    V reg[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int m = t + i * T;
+        V v = c_make<V>(0, 0);
+        if (active && m < n) {
+            v = in[off + m];
+            if (a.conj) v = c_conj(v);
+            if (blue) v = c_mul(v, chirp[m]);
+        }
+        reg[i] = v;
+    }
+    V *slot = lds + c * lds_padded_len(N);
+    int tl = t;
+    __asm__ volatile("" : "+v"(tl));
+    ct.template run<true, kWave, true>(reg, slot, 1, tl);
+    if (blue) {
+        const V *filt = reinterpret_cast<const V *>(K.filt);
+#pragma unroll
+        for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
+        // the second transform's first exchange writes the slot the first one read last: in
+        // order within the wavefront, or behind the pass's leading barrier
+        ct.template run<true, kWave, true>(reg, slot, 1, tl);
+    }
+    const Real sc = a.inverse ? (Real)(1.0 / (double)n) : (Real)1;
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int m = t + i * T;
+            if (m < n) {
+                V v = blue ? c_mul(c_conj(reg[i]), chirp[m]) : reg[i];
+                if (a.conj) v = c_conj(v);
+                out[off + m] = c_scale(v, sc);
+            }
+        }
+    }
+}
+
 template <typename Real, int EMAX>
 __global__ void __launch_bounds__(kRaggedBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) afft_ragged(const RaggedArgs a)
 {
@@ -121,6 +182,19 @@ __global__ void __launch_bounds__(kRaggedBlock) __attribute__((amdgpu_waves_per_
             }
             __syncthreads();
             const RaggedClass &K = sK;
+            if constexpr (sizeof(Real) == 4) {
+                if (K.ctn) {
+                    switch (K.ctn) {
+#define AFFT_RCT(N, E, ...)                                                                      \
+    case N: run_ct<Real, N, E, __VA_ARGS__>(a, K, it, afft_smem); break;
+                        AFFT_RAGGED_CT_LAYOUTS(AFFT_RCT)
+#undef AFFT_RCT
+                    default: break;
+                    }
+                    __syncthreads(); // LDS (transforms and descriptor) reuse by the next item
+                    continue;
+                }
+            }
             if (K.kind == kDirect) {
                 switch (K.E) {
 #define AFFT_RD(EV)                                                                              \

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -98,26 +99,41 @@ bool ragged_layout(int64_t n, int dtype, LdsLayout &L, int maxE)
     return L.npass <= 1 || (size_t)lds_padded_len((int)n) * esz_of(dtype) <= kLdsBudget[dtype];
 }
 
-// The layout of a class of the persistent kernel: the default cap on elements per thread, or the
-// tuned (or forced) one among the caps that give a different valid layout.  `L` is the class's
-// true length (the autotuner's key), `len` the transform length (L, or M for Bluestein).
-bool class_layout(int64_t L, int64_t len, int dtype, int device, LdsLayout &lay)
+// The layout of a class of the persistent kernel: the compiled layout of its transform length
+// (Complex Float lengths of the compiled list; `ct` set), else the pass-table layout under the
+// default cap on elements per thread; or the tuned (or forced) one of these candidates: 0 for
+// the compiled layout, a cap for each pass-table layout that differs.  `L` is the class's true
+// length (the autotuner's key), `len` the transform length (L, or M for Bluestein).
+bool class_layout(int64_t L, int64_t len, int dtype, int device, LdsLayout &lay, bool &ct)
 {
-    if (!ragged_layout(len, dtype, lay, kMaxE[dtype])) return false;
-    std::vector<int> cands = {kMaxE[dtype]};
-    std::vector<int> seen = {lay.E};
-    for (int m : {8, 4, 2}) {
-        LdsLayout l{};
-        if (m < kMaxE[dtype] && ragged_layout(len, dtype, l, m) &&
-            std::find(seen.begin(), seen.end(), l.E) == seen.end()) {
-            cands.push_back(m);
-            seen.push_back(l.E);
+    ct = false;
+    const char *no_ct = std::getenv("AFFT_RAGGED_NO_CT");
+    const bool has_ct = dtype == AFFT_C64 && ragged_ct_elems((int)len) > 0 && !(no_ct && no_ct[0] == '1');
+    const bool has_table = ragged_layout(len, dtype, lay, kMaxE[dtype]);
+    if (!has_ct && !has_table) return false;
+    std::vector<int> cands;
+    if (has_ct) cands.push_back(0);
+    if (has_table) {
+        cands.push_back(kMaxE[dtype]);
+        std::vector<int> seen = {lay.E};
+        for (int m : {8, 4, 2}) {
+            LdsLayout l{};
+            if (m < kMaxE[dtype] && ragged_layout(len, dtype, l, m) &&
+                std::find(seen.begin(), seen.end(), l.E) == seen.end()) {
+                cands.push_back(m);
+                seen.push_back(l.E);
+            }
         }
     }
-    const int m = tune_choice(kTuneRagged, dtype, L, device, cands);
-    if (m > 0 && m != kMaxE[dtype]) {
+    const int t = tune_choice(kTuneRagged, dtype, L, device, cands);
+    const int pick = t >= 0 ? t : cands.front();
+    if (pick == 0) {
+        ct = true;
+        return true;
+    }
+    if (pick != kMaxE[dtype]) {
         LdsLayout l{};
-        if (ragged_layout(len, dtype, l, m)) lay = l;
+        if (ragged_layout(len, dtype, l, pick)) lay = l;
     }
     return true;
 }
@@ -148,12 +164,13 @@ struct Builder {
         LdsLayout lay{};
         const void *tw = nullptr;
         int64_t len = L;
+        bool ct = false;
         if (is_5smooth(L)) {
-            if (!class_layout(L, L, rp.dtype, rp.device, lay)) return false;
+            if (!class_layout(L, L, rp.dtype, rp.device, lay, ct)) return false;
             K.kind = kDirect;
         } else {
             const int64_t M = bluestein_length(L);
-            if (!class_layout(L, M, rp.dtype, rp.device, lay)) return false;
+            if (!class_layout(L, M, rp.dtype, rp.device, lay, ct)) return false;
             K.kind = kBlue;
             K.M = (int)M;
             len = M;
@@ -162,6 +179,21 @@ struct Builder {
                 status = st;
                 return false;
             }
+        }
+        if (ct) {
+            // compiled layout: whole transforms fill the workgroup, twiddles from W_len^m
+            K.ctn = (int)len;
+            K.E = ragged_ct_elems((int)len);
+            const int T = (int)len / K.E;
+            K.C = kRaggedBlock / T;
+            int st = table_power(rp.tables, dev, s, len, 1, len, tw);
+            if (st) {
+                status = st;
+                return false;
+            }
+            K.tw = tw;
+            if (T > 1) lds_need = std::max(lds_need, (size_t)K.C * lds_padded_len((int)len) * esz_of(rp.dtype));
+            return true;
         }
         K.E = lay.E;
         int st = table_pass_twiddles(rp.tables, dev, s, len, lay, K.pt, tw);
@@ -220,7 +252,11 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             if (b.status) return b.status;
             auto lp = std::make_unique<RaggedLong>();
             lp->L = kv.first;
-            lp->small = (int64_t)cnt * kv.first < kSmallLongElems;
+            static const int64_t small_elems = [] {
+                const char *v = std::getenv("AFFT_RAGGED_SMALL");
+                return v ? (int64_t)std::atoll(v) : kSmallLongElems;
+            }();
+            lp->small = (int64_t)cnt * kv.first < small_elems;
             const int64_t ext[2] = {cnt, kv.first};
             // rows packed back to back start at arbitrary 8- or 16-byte offsets
             lp->plan.flags = kPlanUnalignedRows;
@@ -312,7 +348,11 @@ static int run_long(const RaggedLong &lp, int mode, const void *in, void *out, h
 static int ragged_launch(const RaggedPlan &rp, int mode, const void *in, void *out, hipStream_t s,
                          const RaggedExec &x)
 {
-    hipStream_t side = x.side;
+    static const bool serial = [] {
+        const char *v = std::getenv("AFFT_RAGGED_SERIAL");
+        return v && v[0] == '1';
+    }();
+    hipStream_t side = serial ? s : x.side;
     if (hipEventRecord(x.fork, s) != hipSuccess || hipStreamWaitEvent(side, x.fork, 0) != hipSuccess)
         return AFFT_ERROR_HIP;
     auto ws_of = [&](size_t i) {
@@ -490,7 +530,9 @@ extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *plan)
 
 // Tune every length class of a ragged batch on its own: the class's transforms as a ragged plan
 // of one length, its decision points (the persistent kernel's layout, or the layouts inside the
-// batched plan of a long class) timed candidate by candidate.
+// batched plan of a long class) timed candidate by candidate.  A class of the persistent kernel
+// is timed on at least 2^23 elements (its length repeated): a few hundred short transforms take
+// less device time than the host takes to issue the call, which would hide the layout.
 extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *lengths, int device)
 {
     if (dtype != AFFT_C64 && dtype != AFFT_C128) return AFFT_ERROR_UNSUPPORTED_DTYPE;
@@ -515,8 +557,12 @@ extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *len
         if (st) break;
         (void)afft_ragged_plan_destroy(p);
         if (points.empty()) continue;
+        const bool persistent = std::all_of(points.begin(), points.end(),
+                                            [](const TunePoint &q) { return q.kind == kTuneRagged; });
+        const int64_t reps = persistent ? std::max<int64_t>(kv.second, ((int64_t)1 << 23) / kv.first) : kv.second;
+        const std::vector<int64_t> timed((size_t)reps, kv.first);
         void *buf = nullptr;
-        const size_t bytes = (size_t)(kv.first * kv.second) * esz;
+        const size_t bytes = (size_t)(kv.first * reps) * esz;
         if (hipMalloc(&buf, bytes) != hipSuccess || hipMemset(buf, 0, bytes) != hipSuccess) {
             if (buf) (void)hipFree(buf);
             st = AFFT_ERROR_ALLOC;
@@ -524,7 +570,7 @@ extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *len
         }
         st = tune_points(points, device, [&]() -> double {
             afft_ragged_plan *q = nullptr;
-            if (afft_ragged_plan_create(&q, dtype, (int64_t)one.size(), one.data(), device)) return -1;
+            if (afft_ragged_plan_create(&q, dtype, (int64_t)timed.size(), timed.data(), device)) return -1;
             const double us = tune_time_calls(
                 [&](hipStream_t s) { return afft_ragged_exec(q, AFFT_FORWARD, buf, buf, s); }, nullptr);
             (void)afft_ragged_plan_destroy(q);

@@ -111,7 +111,7 @@ bool plausible(int kind, int dtype, int64_t n, int value)
         return C >= 1 && C <= 64 && (C & (C - 1)) == 0 && E >= 1 && n % E == 0;
     }
     case kTuneRegcol: return (value == 1 || value == 2 || value == 4) && regcol_supported(dtype, n);
-    case kTuneRagged: return value >= 1 && value <= 32;
+    case kTuneRagged: return value >= 0 && value <= 32; // 0: the compiled layout
     default: return false;
     }
 }

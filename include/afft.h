@@ -256,8 +256,9 @@ int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w,
  * radices and their order --, the lanes per column of the register kernel) on the device, one
  * decision after the other, and remember the fastest; plans built afterwards use them (the plan
  * cache is cleared).  afft_ragged_autotune does the same for every length class of a ragged batch,
- * each timed alone (elements per thread of the persistent kernel's classes, the layouts of the
- * long classes' plans).  afft_tune_save/afft_tune_load persist the choices as a text file keyed by
+ * each timed alone (for a class of the persistent kernel its compiled layout against the
+ * pass-table layouts of each cap on elements per thread, timed on at least 2^23 elements; the
+ * layouts of the long classes' plans).  afft_tune_save/afft_tune_load persist the choices as a text file keyed by
  * (gfx arch, HIP runtime version, decision kind, dtype, length); stale or malformed lines are
  * ignored. */
 int afft_autotune(int dtype, int rank, const int64_t *extents, int axes, int device);

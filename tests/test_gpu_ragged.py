@@ -109,3 +109,26 @@ def test_ragged_full_configuration(gpu):
     z = plan.execute(Mode.Inverse, y)
     assert O.rel_l2(z.cpu().numpy(), x.cpu().numpy()) < 2e-6
     plan.close()
+
+
+@pytest.mark.parametrize("mode", [Mode.Forward, Mode.Inverse])
+def test_ragged_compiled_and_table_layouts(gpu, monkeypatch, mode):
+    # every length of the persistent kernel's compiled list (Complex Float), the chirp-z classes
+    # whose padded lengths are in it (7 -> 16, 257 -> 640, 1009 -> 2048), partial items (counts
+    # that do not fill a workgroup) and lengths outside the list (15, 100, 243): the compiled
+    # layouts and the pass-table layouts (AFFT_RAGGED_NO_CT=1) each against the oracle
+    rng = np.random.default_rng(12)
+    short = [n for n in SIZE_SET if n <= 4096]
+    lengths = [n for n in short + [7, 15, 100, 243] for _ in range(int(rng.integers(1, 40)))]
+    lengths = [int(n) for n in rng.permutation(lengths)]
+    total = sum(lengths)
+    x = torch.from_numpy(O.uniform(13, (total,))).to(torch.complex64).to(gpu)
+    outs = []
+    for no_ct in ("0", "1"):
+        monkeypatch.setenv("AFFT_RAGGED_NO_CT", no_ct)
+        plan = afft.RaggedPlan(lengths, torch.complex64)
+        y = plan.execute(mode, x)
+        check_rows(y, x, lengths, mode, torch.complex64)
+        outs.append(y)
+        plan.close()
+    assert O.rel_l2(outs[0].cpu().numpy(), outs[1].cpu().numpy()) <= tolerance(False, 4096)
