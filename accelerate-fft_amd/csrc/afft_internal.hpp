@@ -180,6 +180,8 @@ struct RowArgs {
 // LDS bytes, rows per tile, resident workgroups per CU, and the launcher.
 constexpr int kRowTransposed = -2; // the variant index of a length's transposed-store form
 int row_variants(int dtype, int n);
+// true when the default layout of (dtype, n) is a split-exchange one (one row per workgroup)
+bool row_split(int dtype, int n);
 int row_block(int dtype, int n, int variant);
 size_t row_lds_bytes(int dtype, int n, int variant);
 int row_rows_per_tile(int dtype, int n, int variant);

@@ -627,9 +627,11 @@ struct Builder {
     int64_t six_step_n1(int64_t n)
     {
         if (getenv_flag("AFFT_NO_SIX_STEP")) return 0;
+        // (split-exchange rows hold one row per workgroup: too few rows in flight for the
+        // middle pass of a six-step)
         for (int64_t c : {64, 32, 16})
             if (n % c == 0 && regcol_supported(plan.dtype, c) && n / c <= (1 << 30) &&
-                row_variants(plan.dtype, (int)(n / c)) > 0)
+                row_variants(plan.dtype, (int)(n / c)) > 0 && !row_split(plan.dtype, (int)(n / c)))
                 return c;
         return 0;
     }
@@ -638,12 +640,16 @@ struct Builder {
     // o outer blocks; reads slot src, writes slot dst.
     enum class AxisKind { kRows, kSingle, kBluestein, kSplit };
 
-    // How one axis of length n and element stride s_ is transformed; build_plan and add_axis
-    // both follow this decision.
-    AxisKind axis_kind(int64_t n, int64_t s_)
+    // How one axis of length n and element stride s_ is transformed, o outer blocks; build_plan
+    // and add_axis both follow this decision.  Rows beyond 16384 points with a split-exchange
+    // layout run one row per workgroup: a single pass that beats the two-pass six-step on a few
+    // hundred rows (20480 x 447: 59 against 63 us) but not on many (20480 x 1638: 203 against
+    // 188 us), so they take it for up to 4 rows per CU.
+    AxisKind axis_kind(int64_t n, int64_t s_, int64_t o)
     {
         const bool strided = s_ > 1;
-        if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS"))
+        if (!strided && row_variants(plan.dtype, (int)n) > 0 && !getenv_flag("AFFT_NO_ROWS") &&
+            !(row_split(plan.dtype, (int)n) && n > 16384 && o > 4 * (int64_t)dev.cus))
             return AxisKind::kRows;
         if (n <= dev.max_lds_len[plan.dtype] && is_5smooth(n) && !(strided && split_strided(n, s_)))
             return AxisKind::kSingle;
@@ -820,7 +826,7 @@ struct Builder {
         a.st2 = n * s_;
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
-        switch (axis_kind(n, s_)) {
+        switch (axis_kind(n, s_, o)) {
         case AxisKind::kRows: add_rows(n, ntrans, src, cur, pick_row_variant(n)); return cur;
         case AxisKind::kSingle: add_stockham(n, ntrans, a, a, strided, src, cur); return cur;
         case AxisKind::kBluestein: add_bluestein(n, ntrans, a, a, strided, src, cur); return cur;
@@ -927,7 +933,8 @@ int build_plan(Plan &plan, int dtype, int rank, const int64_t *extents, int axes
     {
         int64_t inner = 1;
         for (int ax = rank - 1; ax >= 0; --ax) {
-            if (take(ax) && extents[ax] > 1 && b.axis_kind(extents[ax], inner) == Builder::AxisKind::kSplit)
+            if (take(ax) && extents[ax] > 1 &&
+                b.axis_kind(extents[ax], inner, plan.total / (extents[ax] * inner)) == Builder::AxisKind::kSplit)
                 ++splits;
             inner *= extents[ax];
         }

@@ -28,6 +28,7 @@
 #include <utility>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "afft_device.hpp"
 #include "afft_internal.hpp"
@@ -58,7 +59,16 @@ template <int N, int E, int CM = 1> struct RowShape {
 // minimum rows per tile of the transposed-store form: one 128-byte line of elements
 template <typename Real, bool TR> constexpr int row_min_tile() { return TR ? 64 / (int)sizeof(Real) : 1; }
 
-template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, int... Rs>
+// Position of element p of a row in the one-row array of reals of the split exchange: the low
+// five bits permuted by the next five, so that the stride-R scatters of the early passes (R
+// consecutive lanes' elements R apart) spread over the banks without padding; a bijection on
+// every aligned block of 32, so the array is exactly N reals long.
+__device__ __forceinline__ int lds_swz(int p) { return p ^ ((p >> 5) & 31); }
+
+// SPLIT: rows too long for a complex LDS slot (a row of 20480 points is 160 KB of Complex Float)
+// exchange through one array of N reals, the real parts and then the imaginary parts; one row
+// per workgroup, no prefetch (its registers hold the row).
+template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, bool SPLIT, int... Rs>
 __global__ void __launch_bounds__((RowShape<N, E, row_min_tile<Real, TR>()>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(W, 8)))
 afft_rows(const RowArgs a)
@@ -71,6 +81,7 @@ afft_rows(const RowArgs a)
     constexpr int NP = RL::NP;
     constexpr int PADN = lds_padded_len(N);
     static_assert(T * E == N, "E must divide N");
+    static_assert(!SPLIT || (C == 1 && !TR && !PF && N % 32 == 0), "split exchange: one row, no prefetch");
 
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
@@ -205,7 +216,34 @@ afft_rows(const RowArgs a)
 #pragma unroll
                 for (int r = 0; r < R; ++r) reg[j + r * B] = v[r];
             });
-            if constexpr (p + 1 < NP) {
+            if constexpr (p + 1 < NP && SPLIT) {
+                Real *fl = reinterpret_cast<Real *>(afft_smem);
+                // real parts, then imaginary parts: the second round scatters the .y halves,
+                // which the first round's reads (into .x) left as they were.  The positions are
+                // recomputed per round from the thread index (through an empty asm): kept live
+                // across the passes they would not fit the register budget of 1024 threads.
+                static_for<2>([&](auto hc) {
+                    constexpr int h = decltype(hc)::value;
+                    int tp = tl;
+                    __asm__ volatile("" : "+v"(tp));
+                    lds_sync(); // the array's previous contents have been read
+#pragma unroll
+                    for (int j = 0; j < B; ++j) {
+                        const int b = tp + j * T;
+                        const int k = b % Ns;
+                        const int dst = (b - k) * R + k;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) fl[lds_swz(dst + r * Ns)] = h ? reg[j + r * B].y : reg[j + r * B].x;
+                    }
+                    lds_sync();
+#pragma unroll
+                    for (int i = 0; i < E; ++i) {
+                        const Real v = fl[lds_swz(tp + i * T)];
+                        if constexpr (h) reg[i].y = v;
+                        else reg[i].x = v;
+                    }
+                });
+            } else if constexpr (p + 1 < NP) {
                 row_sync<T>(); // the previous exchange has been read by every thread of the row
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
@@ -285,16 +323,22 @@ struct RowEntry {
     RowFn fn;
     int block;
     bool tr;
-    int rows; // rows per tile
+    int rows;   // rows per tile
+    bool split; // exchanges through N reals (one row per workgroup)
 };
 
 #define AFFT_ROW(REAL, DT, N, E, PF, W, LD, ...)                                                 \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, false, \
-             RowShape<N, E>::C}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, \
+             false, RowShape<N, E>::C, false}
 // the transposed-store form, one per length (selected with variant kRowTransposed)
 #define AFFT_ROW_TR(REAL, DT, N, E, PF, W, LD, ...)                                              \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, __VA_ARGS__>,                  \
-             RowShape<N, E, row_min_tile<REAL, true>()>::BLOCK, true, RowShape<N, E, row_min_tile<REAL, true>()>::C}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, false, __VA_ARGS__>,           \
+             RowShape<N, E, row_min_tile<REAL, true>()>::BLOCK, true, RowShape<N, E, row_min_tile<REAL, true>()>::C, \
+             false}
+// rows beyond a complex LDS slot: the split exchange
+#define AFFT_ROW_SPLIT(REAL, DT, N, E, W, LD, ...)                                               \
+    RowEntry{DT, N, E, false, afft_rows<REAL, N, E, false, W, LD, false, true, __VA_ARGS__>,     \
+             RowShape<N, E>::BLOCK, false, 1, true}
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
@@ -315,6 +359,12 @@ const RowEntry kRows[] = {
     AFFT_ROW(float, 0, 10240, 20, true, 2, true, 2, 4, 4, 4, 4, 4, 5),
     AFFT_ROW(float, 0, 12288, 24, true, 2, true, 8, 8, 8, 8, 3),
     AFFT_ROW(float, 0, 12288, 24, false, 2, true, 8, 8, 8, 8, 3),
+    // one row per workgroup through the split exchange: 16384 and 20480, whose rows fit the 128
+    // registers of 1024 threads (24576, 32768 and 40960 did not: their spills made them slower
+    // than two passes); every pass but the last has stride <= T
+    AFFT_ROW_SPLIT(float, 0, 16384, 16, 4, true, 16, 16, 4, 16),
+    AFFT_ROW_SPLIT(float, 0, 20480, 20, 4, true, 4, 4, 4, 4, 4, 4, 5),
+
     // the row lengths of six-step transforms n = 64 * n2
     AFFT_ROW(float, 0, 160, 20, true, 2, true, 2, 4, 4, 5),
     AFFT_ROW(float, 0, 320, 20, true, 2, true, 4, 4, 4, 5),
@@ -338,6 +388,17 @@ const RowEntry kRows[] = {
     AFFT_ROW(double, 1, 4096, 8, false, 2, false, 8, 8, 8, 8),
 };
 
+// AFFT_NO_ROW_SPLIT=1 hides the split-exchange rows (their lengths then take the multi-pass
+// plans)
+bool row_usable(const RowEntry &e)
+{
+    static const bool no_split = [] {
+        const char *v = std::getenv("AFFT_NO_ROW_SPLIT");
+        return v && v[0] == '1';
+    }();
+    return !(e.split && no_split);
+}
+
 const RowEntry *find_row(int dtype, int n, int variant)
 {
     if (variant == kRowTransposed) {
@@ -348,7 +409,7 @@ const RowEntry *find_row(int dtype, int n, int variant)
     const RowEntry *first = nullptr;
     int seen = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n && !e.tr) {
+        if (e.dtype == dtype && e.n == n && !e.tr && row_usable(e)) {
             if (!first) first = &e;
             if (seen++ == variant) return &e;
         }
@@ -361,8 +422,14 @@ int row_variants(int dtype, int n)
 {
     int v = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n && !e.tr) ++v;
+        if (e.dtype == dtype && e.n == n && !e.tr && row_usable(e)) ++v;
     return v;
+}
+
+bool row_split(int dtype, int n)
+{
+    const RowEntry *e = find_row(dtype, n, 0);
+    return e && e->split;
 }
 
 int row_block(int dtype, int n, int variant)
@@ -375,6 +442,7 @@ size_t row_lds_bytes(int dtype, int n, int variant)
 {
     const RowEntry *e = find_row(dtype, n, variant);
     if (!e) return 0;
+    if (e->split) return (size_t)e->n * (dtype == 0 ? 4 : 8);
     const int C = e->rows;
     size_t elems = (size_t)C * lds_padded_len(e->n);
     if (e->tr) elems = std::max(elems, (size_t)e->n * (size_t)(C + 1)); // the regroup buffer

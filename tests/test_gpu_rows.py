@@ -14,13 +14,16 @@ from conftest import tolerance
 pytestmark = pytest.mark.gpu
 
 LENGTHS = [(torch.complex64, n) for n in (256, 512, 1024, 2048, 4096, 8192, 5120, 6144, 10240, 12288,
-                                          160, 320, 384, 640, 768, 1536)] + \
+                                          160, 320, 384, 640, 768, 1536,
+                                          16384, 20480)] + \
           [(torch.complex128, n) for n in (256, 512, 1024, 2048, 4096)]
 
 
 @pytest.mark.parametrize("dtype,n", LENGTHS)
 def test_every_row_variant(gpu, dtype, n, monkeypatch):
-    rows = max(3000, (1 << 23) // n) + 5  # several persistent iterations and a partial tile
+    # several persistent iterations and a partial tile (rows beyond 16384 points take the row
+    # kernel for up to 4 rows per CU, a six-step beyond)
+    rows = (1000 if n > 16384 else max(3000, (1 << 23) // n)) + 5
     x = torch.from_numpy(O.uniform(n, (rows, n))).to(dtype).to(gpu)
     xr = x.cpu().numpy().astype(np.complex128)
     pick = np.r_[0:40, rows - 40:rows]
@@ -43,7 +46,7 @@ def test_autotune_round_trip(gpu, tmp_path):
     # pass (the split axis of fft2D 4096^2): every decision point of each plan is timed
     assert lib.afft_autotune(0, 2, afft._lib.extents_arg([64, 4096]), 0, 0) == 0
     assert lib.afft_autotune(0, 3, afft._lib.extents_arg([16, 512, 512]), 2, 0) == 0
-    assert lib.afft_autotune(0, 2, afft._lib.extents_arg([1024, 4096]), 1, 0) == 0
+    assert lib.afft_autotune(0, 2, afft._lib.extents_arg([4096, 4096]), 1, 0) == 0
     # a ragged batch: every class timed alone (persistent-kernel classes and a long class)
     lengths = [64] * 300 + [1000] * 60 + [4096] * 30 + [257] * 40 + [20480] * 8
     arr = (ctypes.c_int64 * len(lengths))(*lengths)
