@@ -35,11 +35,13 @@ struct RaggedLong {
     bool small = false;            // too few elements to fill the device: runs on the side stream
 };
 
-// A long class below this many elements (64 MiB of FP32 data) is latency- or tail-bound: its one
+// A long class below this many elements (32 MiB of FP32 data) is latency- or tail-bound: its one
 // or two launches hold a few rows per CU.  Those classes run on the side stream beside the large
-// ones (measured on the 10,000-transform batch: 1M -> 8M moved the 5120...12288 classes there and
-// took it from 550 to 465 us).
-constexpr int64_t kSmallLongElems = 1 << 23;
+// ones, and the threshold balances the two streams (measured on the 10,000-transform batch:
+// 1M -> 8M moved the 5120...12288 classes to the side stream, 550 -> 465 us; once the large
+// classes took fewer passes, 8M -> 4M moved 12288 x 432 back, 392 -> 368 us, while 2.8M, which
+// also moves 8192 x 357, gave 400 us).
+constexpr int64_t kSmallLongElems = 1 << 22;
 
 // The per-execution resources of a ragged batch: a side stream forked from and joined back to
 // the caller's stream, with its two events, and one device block holding the workspace of every
