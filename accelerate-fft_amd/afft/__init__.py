@@ -285,5 +285,25 @@ def spectral_conv_vjp(x, H, gy, want_x=True, want_H=True, spectrum=None):
     return gx, gH
 
 
+def spectral_conv_vjp_residual(x, H, y, t, alpha=2.0, want_x=True, want_H=True, spectrum=None):
+    """(gx, gH) for the squared-error loss L = (alpha/2) sum |y - t|^2 of y = spectral_conv(x, H):
+    spectral_conv_vjp with gy = alpha (y - t), formed as the first row pass loads y and t (gy is
+    never stored).  F x is recomputed from x, or taken from `spectrum` (x may then be None)."""
+    H = _device_tensor(H)
+    y = _device_tensor(y)
+    t = _device_tensor(t)
+    src = _device_tensor(spectrum if spectrum is not None else x)
+    if src.shape != y.shape or t.shape != y.shape or src.dtype != y.dtype or t.dtype != y.dtype:
+        raise ValueError("x (or the spectrum), y and t must have the same shape and dtype")
+    gx = torch.empty_like(y) if want_x else None
+    gH = torch.empty_like(H) if want_H else None
+    with torch.cuda.device(y.device):
+        L.check(L.lib().afft_spectral_conv_vjp_residual(
+            _dtype_code(y), y.shape[0], y.shape[1], y.shape[2], _ptr(src), 1 if spectrum is not None else 0,
+            _ptr(H), _ptr(y), _ptr(t), float(alpha), _ptr(gx) if gx is not None else None,
+            _ptr(gH) if gH is not None else None, _stream(y.device)), "afft_spectral_conv_vjp_residual")
+    return gx, gH
+
+
 from . import centre  # noqa: E402  (DFT/Centre.hs mirror; uses the helpers above)
 from . import dft  # noqa: E402  (Math/DFT.hs mirror)

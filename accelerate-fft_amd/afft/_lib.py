@@ -71,6 +71,7 @@ def lib():
     _sig(L.afft_spectral_conv_vjp, i, i, i64, i64, i64, vp, vp, vp, vp, vp, vp)
     _sig(L.afft_spectral_conv_fwd_save, i, i, i64, i64, i64, vp, vp, vp, vp, vp)
     _sig(L.afft_spectral_conv_vjp_saved, i, i, i64, i64, i64, vp, vp, vp, vp, vp, vp)
+    _sig(L.afft_spectral_conv_vjp_residual, i, i, i64, i64, i64, vp, i, vp, vp, vp, ctypes.c_double, vp, vp, vp)
     _sig(L.afft_autotune, i, i, i, pi64, i, i)
     _sig(L.afft_ragged_autotune, i, i, i64, pi64, i)
     _sig(L.afft_tune_save, i, ctypes.c_char_p)

@@ -83,6 +83,18 @@ def san_runtime():
     return os.path.realpath(hits[-1]) if hits else None
 
 
+def sanitized_stale():
+    """True when lib_san/libafft.so is missing or older than a source, a header or the regular
+    library it links the kernels of."""
+    if not os.path.exists(SAN_LIB):
+        return True
+    deps = [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [os.path.join(HERE, "..", "include", "afft.h")]
+    if os.path.exists(LIB):
+        deps.append(LIB)
+    newest = max(os.path.getmtime(d) for d in deps if os.path.isfile(d))
+    return os.path.getmtime(SAN_LIB) < newest
+
+
 def build_sanitized(verbose=True):
     rt = san_runtime()
     if rt is None:

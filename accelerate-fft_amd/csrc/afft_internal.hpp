@@ -174,11 +174,15 @@ struct RowArgs {
     // k1 = (u - s_o) mod n1, s_o = the element address of X_o[0] modulo C, so that each tile's
     // stores fill aligned runs of C elements wherever the transform starts
     int tr_rot;
+    // residual loads (variant kRowResidual): the rows are alpha (in - in2), in2 laid out as in
+    const void *in2;
+    double alpha;
 };
 
 // afft_rows.hip: number of compiled variants for (dtype, n) (0 = none), their launch shape,
 // LDS bytes, rows per tile, resident workgroups per CU, and the launcher.
 constexpr int kRowTransposed = -2; // the variant index of a length's transposed-store form
+constexpr int kRowResidual = -3;   // the variant index of a length's residual-load form
 int row_variants(int dtype, int n);
 // true when the default layout of (dtype, n) is a split-exchange one (one row per workgroup)
 bool row_split(int dtype, int n);

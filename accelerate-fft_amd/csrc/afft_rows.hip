@@ -68,7 +68,9 @@ __device__ __forceinline__ int lds_swz(int p) { return p ^ ((p >> 5) & 31); }
 // SPLIT: rows too long for a complex LDS slot (a row of 20480 points is 160 KB of Complex Float)
 // exchange through one array of N reals, the real parts and then the imaginary parts; one row
 // per workgroup, no prefetch (its registers hold the row).
-template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, bool SPLIT, int... Rs>
+// RES: the rows loaded are alpha (in - in2), the difference of two arrays (the residual of a
+// squared-error loss, transformed without being stored first).
+template <typename Real, int N, int E, bool PF, int W, bool LD, bool TR, bool SPLIT, bool RES, int... Rs>
 __global__ void __launch_bounds__((RowShape<N, E, row_min_tile<Real, TR>()>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(W, 8)))
 afft_rows(const RowArgs a)
@@ -126,16 +128,27 @@ afft_rows(const RowArgs a)
         return row;
     };
     V nxt[E];
+    V nxt2[RES ? E : 1];
+    const V *__restrict__ in2 = reinterpret_cast<const V *>(a.in2);
     auto load_tile = [&](int64_t tile) {
         if (tile < ntiles && tile * C + c < nrows) {
             const int64_t row = src_row(tile * C + c);
-            const V *p = in + (roffs ? roffs[row] : row * N) + t;
+            const int64_t ro = (roffs ? roffs[row] : row * N) + t;
+            const V *p = in + ro;
             if (nt_in) {
 #pragma unroll
                 for (int i = 0; i < E; ++i) nxt[i] = c_load_nt(p + i * T);
+                if constexpr (RES) {
+#pragma unroll
+                    for (int i = 0; i < E; ++i) nxt2[i] = c_load_nt(in2 + ro + i * T);
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < E; ++i) nxt[i] = p[i * T];
+                if constexpr (RES) {
+#pragma unroll
+                    for (int i = 0; i < E; ++i) nxt2[i] = in2[ro + i * T];
+                }
             }
         }
     };
@@ -178,7 +191,14 @@ afft_rows(const RowArgs a)
         if constexpr (LD) __asm__ volatile("" : "+v"(tl));
         V reg[E];
 #pragma unroll
-        for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
+        for (int i = 0; i < E; ++i) {
+            V v = nxt[i];
+            if constexpr (RES) {
+                const Real al = (Real)a.alpha;
+                v = c_make<V>(al * (v.x - nxt2[i].x), al * (v.y - nxt2[i].y));
+            }
+            reg[i] = cin ? c_conj(v) : v;
+        }
         if (a.sg_in.on && gt * C + c < nrows) {
             const int64_t r0 = src_row(gt * C + c);
             const int64_t rb = roffs ? roffs[r0] : r0 * N;
@@ -325,20 +345,25 @@ struct RowEntry {
     bool tr;
     int rows;   // rows per tile
     bool split; // exchanges through N reals (one row per workgroup)
+    bool res;   // loads alpha (in - in2) (selected with variant kRowResidual)
 };
 
 #define AFFT_ROW(REAL, DT, N, E, PF, W, LD, ...)                                                 \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, false, __VA_ARGS__>, RowShape<N, E>::BLOCK, \
-             false, RowShape<N, E>::C, false}
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, false, false, __VA_ARGS__>,   \
+             RowShape<N, E>::BLOCK, false, RowShape<N, E>::C, false, false}
 // the transposed-store form, one per length (selected with variant kRowTransposed)
 #define AFFT_ROW_TR(REAL, DT, N, E, PF, W, LD, ...)                                              \
-    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, false, __VA_ARGS__>,           \
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, true, false, false, __VA_ARGS__>,    \
              RowShape<N, E, row_min_tile<REAL, true>()>::BLOCK, true, RowShape<N, E, row_min_tile<REAL, true>()>::C, \
-             false}
+             false, false}
 // rows beyond a complex LDS slot: the split exchange
 #define AFFT_ROW_SPLIT(REAL, DT, N, E, W, LD, ...)                                               \
-    RowEntry{DT, N, E, false, afft_rows<REAL, N, E, false, W, LD, false, true, __VA_ARGS__>,     \
-             RowShape<N, E>::BLOCK, false, 1, true}
+    RowEntry{DT, N, E, false, afft_rows<REAL, N, E, false, W, LD, false, true, false, __VA_ARGS__>, \
+             RowShape<N, E>::BLOCK, false, 1, true, false}
+// the residual-load form, one per length (the first row pass of the squared-error spectral VJP)
+#define AFFT_ROW_RES(REAL, DT, N, E, PF, W, LD, ...)                                             \
+    RowEntry{DT, N, E, PF, afft_rows<REAL, N, E, PF, W, LD, false, false, true, __VA_ARGS__>,    \
+             RowShape<N, E>::BLOCK, false, RowShape<N, E>::C, false, true}
 
 // The first entry of a (dtype, n) is the default; the others are tuning alternatives.
 const RowEntry kRows[] = {
@@ -386,6 +411,12 @@ const RowEntry kRows[] = {
     AFFT_ROW(double, 1, 2048, 8, false, 2, false, 8, 8, 8, 4),
     AFFT_ROW(double, 1, 4096, 8, true, 2, false, 8, 8, 8, 8),
     AFFT_ROW(double, 1, 4096, 8, false, 2, false, 8, 8, 8, 8),
+    // residual loads, with the default layouts of the image widths of spectral convolutions
+    AFFT_ROW_RES(float, 0, 256, 16, true, 2, false, 16, 16),
+    AFFT_ROW_RES(float, 0, 512, 16, true, 2, false, 16, 16, 2),
+    AFFT_ROW_RES(float, 0, 1024, 16, true, 2, false, 16, 16, 4),
+    AFFT_ROW_RES(double, 1, 256, 8, true, 2, false, 8, 8, 4),
+    AFFT_ROW_RES(double, 1, 512, 8, true, 2, false, 8, 8, 8),
 };
 
 // AFFT_NO_ROW_SPLIT=1 hides the split-exchange rows (their lengths then take the multi-pass
@@ -401,15 +432,15 @@ bool row_usable(const RowEntry &e)
 
 const RowEntry *find_row(int dtype, int n, int variant)
 {
-    if (variant == kRowTransposed) {
+    if (variant == kRowTransposed || variant == kRowResidual) {
         for (const RowEntry &e : kRows)
-            if (e.dtype == dtype && e.n == n && e.tr) return &e;
+            if (e.dtype == dtype && e.n == n && (variant == kRowTransposed ? e.tr : e.res)) return &e;
         return nullptr;
     }
     const RowEntry *first = nullptr;
     int seen = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n && !e.tr && row_usable(e)) {
+        if (e.dtype == dtype && e.n == n && !e.tr && !e.res && row_usable(e)) {
             if (!first) first = &e;
             if (seen++ == variant) return &e;
         }
@@ -422,7 +453,7 @@ int row_variants(int dtype, int n)
 {
     int v = 0;
     for (const RowEntry &e : kRows)
-        if (e.dtype == dtype && e.n == n && !e.tr && row_usable(e)) ++v;
+        if (e.dtype == dtype && e.n == n && !e.tr && !e.res && row_usable(e)) ++v;
     return v;
 }
 

@@ -204,7 +204,56 @@ __global__ void afft_reduce_groups(typename cplx_of<Real>::type *out, const type
     }
 }
 
+// out[k] = alpha (y[k] - t[k]): the residual of a squared-error loss, for the paths that need
+// the upstream gradient stored
+template <typename Real>
+__global__ void afft_residual(typename cplx_of<Real>::type *out, const typename cplx_of<Real>::type *y,
+                              const typename cplx_of<Real>::type *t, Real alpha, int64_t n)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const V a = y[k], b = t[k];
+        out[k] = c_make<V>(alpha * (a.x - b.x), alpha * (a.y - b.y));
+    }
+}
+
 namespace {
+
+hipError_t launch_residual(int dtype, void *out, const void *y, const void *t, double alpha, int64_t n, hipStream_t s)
+{
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256));
+    if (dtype == AFFT_C64)
+        hipLaunchKernelGGL(afft_residual<float>, dim3((unsigned)g), dim3(256), 0, s, (float2 *)out,
+                           (const float2 *)y, (const float2 *)t, (float)alpha, n);
+    else
+        hipLaunchKernelGGL(afft_residual<double>, dim3((unsigned)g), dim3(256), 0, s, (double2 *)out,
+                           (const double2 *)y, (const double2 *)t, alpha, n);
+    return hipGetLastError();
+}
+
+// The forward row transforms of alpha (y - t) into out: the row plan's single row launch in its
+// residual-load form, which reads y and t and never stores the difference; else the difference
+// stored into out, then transformed in place.
+int residual_rows(const Plan &rp, int dtype, const void *y, const void *t, double alpha, void *out, hipStream_t s)
+{
+    if (rp.launches.size() == 1 && rp.launches[0].kind == Launch::kRows &&
+        row_block(dtype, rp.launches[0].row_n, kRowResidual) == row_block(dtype, rp.launches[0].row_n, rp.launches[0].row_variant) &&
+        row_block(dtype, rp.launches[0].row_n, kRowResidual) > 0) {
+        const Launch &l = rp.launches[0];
+        RowArgs a = l.ra;
+        a.in = y;
+        a.in2 = t;
+        a.alpha = alpha;
+        a.out = out;
+        a.conj_in = a.conj_out = 0;
+        a.scale = 1.0;
+        a.nt_in = 1;  // y and t are read once
+        a.nt_out = 0; // the column kernel reads the result next
+        return launch_rows(dtype, l.row_n, kRowResidual, a, l.grid, s) == hipSuccess ? AFFT_SUCCESS : AFFT_ERROR_HIP;
+    }
+    if (launch_residual(dtype, out, y, t, alpha, rp.total, s) != hipSuccess) return AFFT_ERROR_HIP;
+    return run_plan(rp, -1, out, out, 1.0, s);
+}
 
 #define AFFT_SPEC_E(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16)
 
@@ -428,16 +477,19 @@ extern "C" int afft_spectral_conv_fwd_save(int dtype, int64_t batch, int64_t h, 
     return conv_fwd(dtype, batch, h, w, d_x, d_H, d_y, d_spec, stream);
 }
 
-// d_x is x, or F x when x_is_spec
+// d_x is x, or F x when x_is_spec.  The upstream gradient is d_gy, or (residual form, d_t not
+// null) alpha (d_y - d_t), formed as the first row pass loads y and t.
 static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x, bool x_is_spec,
-                    const void *d_H, const void *d_gy, void *d_gx, void *d_gH, void *stream)
+                    const void *d_H, const void *d_gy, void *d_gx, void *d_gH, void *stream,
+                    const void *d_y = nullptr, const void *d_t = nullptr, double alpha = 0.0)
 {
     int st = check_args(dtype, batch, h, w);
     if (st) return st;
     const size_t esz = dtype == AFFT_C64 ? 8 : 16;
     hipStream_t s = (hipStream_t)stream;
+    const bool residual = d_t != nullptr;
     if (h * w == 0) return AFFT_SUCCESS;
-    if (!d_x || !d_H || !d_gy || (!d_gx && !d_gH)) return AFFT_ERROR_INVALID_VALUE;
+    if (!d_x || !d_H || (residual ? !d_y : !d_gy) || (!d_gx && !d_gH)) return AFFT_ERROR_INVALID_VALUE;
     if (x_is_spec && (d_x == d_gx || d_x == d_gH)) return AFFT_ERROR_INVALID_VALUE;
     if (batch == 0) {
         if (d_gH && hipMemsetAsync(d_gH, 0, (size_t)(h * w) * esz, s) != hipSuccess) return AFFT_ERROR_HIP;
@@ -446,7 +498,18 @@ static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *
     int dev;
     if ((st = current_device(dev))) return st;
     ColShape cs;
-    if (!col_shape(dtype, h, cs, true)) return conv_vjp_unfused(dtype, batch, h, w, d_x, x_is_spec, d_H, d_gy, d_gx, d_gH, dev, s);
+    if (!col_shape(dtype, h, cs, true)) {
+        if (!residual)
+            return conv_vjp_unfused(dtype, batch, h, w, d_x, x_is_spec, d_H, d_gy, d_gx, d_gH, dev, s);
+        // the unfused path takes the upstream gradient stored
+        void *gy = nullptr;
+        const int64_t n = batch * h * w;
+        if (hipMallocAsync(&gy, (size_t)n * esz, s) != hipSuccess) return AFFT_ERROR_ALLOC;
+        if (launch_residual(dtype, gy, d_y, d_t, alpha, n, s) != hipSuccess) st = AFFT_ERROR_HIP;
+        if (!st) st = conv_vjp_unfused(dtype, batch, h, w, d_x, x_is_spec, d_H, gy, d_gx, d_gH, dev, s);
+        (void)hipFreeAsync(gy, s);
+        return st;
+    }
     const ColTables *ct = nullptr;
     if ((st = col_tables(dtype, h, dev, cs, ct))) return st;
     const int64_t rows[2] = {batch * h, w};
@@ -470,7 +533,7 @@ static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *
     void *partial = base + xr_bytes;
     void *gr = d_gx ? d_gx : (void *)(base + xr_bytes + part);
     if (!x_is_spec) st = run_plan(*rp, -1, d_x, const_cast<void *>(xr), 1.0, stream);
-    if (!st) st = run_plan(*rp, -1, d_gy, gr, 1.0, stream);
+    if (!st) st = residual ? residual_rows(*rp, dtype, d_y, d_t, alpha, gr, s) : run_plan(*rp, -1, d_gy, gr, 1.0, stream);
     if (!st) {
         ColConvArgs a{};
         a.data = gr;
@@ -513,6 +576,14 @@ extern "C" int afft_spectral_conv_vjp(int dtype, int64_t batch, int64_t h, int64
                                       void *stream)
 {
     return conv_vjp(dtype, batch, h, w, d_x, false, d_H, d_gy, d_gx, d_gH, stream);
+}
+
+extern "C" int afft_spectral_conv_vjp_residual(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                               int x_is_spectrum, const void *d_H, const void *d_y,
+                                               const void *d_t, double alpha, void *d_gx, void *d_gH, void *stream)
+{
+    if (!d_t && batch * h * w != 0) return AFFT_ERROR_INVALID_VALUE;
+    return conv_vjp(dtype, batch, h, w, d_x, x_is_spectrum != 0, d_H, nullptr, d_gx, d_gH, stream, d_y, d_t, alpha);
 }
 
 extern "C" int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_spec,

@@ -380,6 +380,14 @@ def spectral_extras(afft, Mode, torch, dev, args, chk):
         gy = 2 * (y - t)
         return y, afft.spectral_conv_vjp(None, H, gy, spectrum=spec)
 
+    def fused_residual():  # the VJP forms gy = 2 (y - t) as it loads y and t
+        y = afft.spectral_conv(x, H)
+        return y, afft.spectral_conv_vjp_residual(x, H, y, t, 2.0)
+
+    def fused_residual_saved():
+        y, spec = afft.spectral_conv(x, H, save_spectrum=True)
+        return y, afft.spectral_conv_vjp_residual(None, H, y, t, 2.0, spectrum=spec)
+
     def plain():
         ys = []
         for b in range(B):
@@ -396,6 +404,8 @@ def spectral_extras(afft, Mode, torch, dev, args, chk):
 
     tf = time_calls(fused, max(3, args.steps // 10), 2)
     tfs = time_calls(fused_saved, max(3, args.steps // 10), 2)
+    tfr = time_calls(fused_residual, max(3, args.steps // 10), 2)
+    tfrs = time_calls(fused_residual_saved, max(3, args.steps // 10), 2)
     tp = time_calls(plain, 3, 1)
     tfwd = time_calls(lambda: afft.spectral_conv(x, H), max(3, args.steps // 5), 2)
     res = {"spectral_conv_fused_fwd_vjp": {"us": round(tf * 1e6, 1)},
@@ -403,10 +413,17 @@ def spectral_extras(afft, Mode, torch, dev, args, chk):
                                             "hbm_frac_per_pass_3": round(3 * 2 * 8.0 * B * M / tfwd / 1e9 / HBM_PEAK_GBS, 4)},
            "spectral_conv_fused_fwd_vjp_saved": {"us": round(tfs * 1e6, 1),
                                                  "note": "forward stores F x, VJP reads it"},
+           "spectral_conv_fused_fwd_vjp_residual": {"us": round(tfr * 1e6, 1),
+                                                    "note": "the VJP forms gy = 2 (y - t) on load"},
+           "spectral_conv_fused_fwd_vjp_residual_saved": {"us": round(tfrs * 1e6, 1),
+                                                          "note": "stored F x and gy formed on load"},
            "spectral_conv_plain_fwd_vjp": {"us": round(tp * 1e6, 1)},
            "spectral_conv_speedup_fwd_vjp": round(tp / tf, 2),
-           "spectral_conv_speedup_fwd_vjp_saved": round(tp / tfs, 2)}
-    for key, fn in (("spectral_conv_fused_fwd_vjp", fused), ("spectral_conv_fused_fwd_vjp_saved", fused_saved)):
+           "spectral_conv_speedup_fwd_vjp_saved": round(tp / tfs, 2),
+           "spectral_conv_speedup_fwd_vjp_residual": round(tp / tfr, 2)}
+    for key, fn in (("spectral_conv_fused_fwd_vjp", fused), ("spectral_conv_fused_fwd_vjp_saved", fused_saved),
+                    ("spectral_conv_fused_fwd_vjp_residual", fused_residual),
+                    ("spectral_conv_fused_fwd_vjp_residual_saved", fused_residual_saved)):
         y, (gx, gH) = fn()
         res[key].update(chk.spectral(xh, Hh, th, y.cpu().numpy(), gx.cpu().numpy(), gH.cpu().numpy()))
     gxp, gHp = plain()

@@ -251,6 +251,17 @@ int afft_spectral_conv_vjp_saved(int dtype, int64_t batch, int64_t h, int64_t w,
                                  const void *d_H, const void *d_gy, void *d_gx, void *d_gH,
                                  void *stream);
 
+/* The VJP of a squared-error loss L = (alpha / 2) sum_b |y_b - t_b|^2 through the convolution,
+ * i.e. with dL/dy = alpha (d_y - d_t), which the first row pass forms as it loads y and t: the
+ * upstream gradient is never stored (alpha = 2 for sum |y - t|^2).  d_x is x, or with
+ * x_is_spectrum = 1 the stored spectrum F x (as for afft_spectral_conv_vjp_saved).  Outputs as
+ * afft_spectral_conv_vjp.  No counterpart in the reference API (an extension of the adjoint
+ * kernels for training loops, like the two entry points above). */
+int afft_spectral_conv_vjp_residual(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+                                    int x_is_spectrum, const void *d_H, const void *d_y,
+                                    const void *d_t, double alpha, void *d_gx, void *d_gH,
+                                    void *stream);
+
 /* Autotuning: time the candidate layouts of every decision the planner makes for a shape (the row
  * kernel variant, elements per thread and columns per workgroup of strided tiles -- which fix the
  * radices and their order --, the lanes per column of the register kernel) on the device, one

@@ -98,6 +98,30 @@ def test_spectral_conv_saved_spectrum(gpu, dtype, shape):
     assert none is None and O.rel_l2(host(gH2), wH) <= tol * 4
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+# w = 256 and 512 (and 1024 in FP32) take the residual-load row kernel, the others the stored
+# difference; h = 14 is the unfused composition
+@pytest.mark.parametrize("shape", [(4, 64, 512), (3, 256, 256), (2, 32, 1024), (5, 64, 96), (3, 14, 10)])
+def test_spectral_conv_vjp_residual(gpu, dtype, shape):
+    """The squared-error VJP (gy = alpha (y - t) formed on load) against the oracle's VJP of the
+    same gy, from x and from the stored spectrum."""
+    B, h, w = shape
+    x, H, t = O.normal(23, shape), O.normal(24, (h, w)), O.normal(25, shape)
+    xd, Hd, td = dev(x, dtype, gpu), dev(H, dtype, gpu), dev(t, dtype, gpu)
+    tol = tolerance(dtype == torch.complex128, h * w)
+    y, spec = afft.spectral_conv(xd, Hd, save_spectrum=True)
+    for alpha in (2.0, -0.75):
+        gy = alpha * (host(y) - host(td))
+        wx, wH = oracle_vjp(host(xd), host(Hd), gy)
+        for kw in ({"x": xd}, {"x": None, "spectrum": spec}):
+            gx, gH = afft.spectral_conv_vjp_residual(kw["x"], Hd, y, td, alpha, spectrum=kw.get("spectrum"))
+            assert O.rel_l2(host(gx), wx) <= tol, (alpha, kw.keys())
+            assert O.rel_l2(host(gH), wH) <= tol * 4, (alpha, kw.keys())
+    none, gH2 = afft.spectral_conv_vjp_residual(xd, Hd, y, td, 2.0, want_x=False)
+    assert none is None
+    assert O.rel_l2(host(gH2), oracle_vjp(host(xd), host(Hd), 2.0 * (host(y) - host(td)))[1]) <= tol * 4
+
+
 def test_spectral_saved_spectrum_rejects_aliasing(gpu):
     lib = afft._lib.lib()
     x = torch.zeros(2, 16, 16, dtype=torch.complex64, device=gpu)

@@ -18,7 +18,7 @@ def sanitized_env():
     if rt is None:
         pytest.skip("no clang ASan runtime in this image")
     lib = build.SAN_LIB
-    if not os.path.exists(lib):
+    if build.sanitized_stale():
         lib = build.build_sanitized(verbose=False)
     env = dict(os.environ)
     env.update({"AFFT_LIB": lib, "LD_PRELOAD": rt,
