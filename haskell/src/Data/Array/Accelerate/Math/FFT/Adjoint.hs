@@ -28,6 +28,7 @@ module Data.Array.Accelerate.Math.FFT.Adjoint (
 
   spectralConv,
   spectralConvVJP,
+  spectralConvVJPResidual,
 
 ) where
 
@@ -167,6 +168,33 @@ spectralConvVJP x h gy
                         in  convolveVJP x' h' g'
     in
     go (A.lift (x, h, gy))
+
+-- | 'spectralConvVJP' for the squared-error loss L = (alpha/2) sum |y - t|^2, from y = spectralConv
+-- x H and the target t: the gradient alpha (y - t) is formed by the HIP backend as its first row
+-- pass loads y and t, and never stored.
+spectralConvVJPResidual :: forall e. Numeric e
+                        => Double
+                        -> Acc (Array DIM3 (Complex e))
+                        -> Acc (Array DIM2 (Complex e))
+                        -> Acc (Array DIM3 (Complex e))
+                        -> Acc (Array DIM3 (Complex e))
+                        -> Acc (Array DIM3 (Complex e), Array DIM2 (Complex e))
+spectralConvVJPResidual alpha x h y t
+  = let
+        eR    = numericR @e
+        go    =
+#ifdef ACCELERATE_LLVM_HIP_BACKEND
+                  foreignAcc (HIP.spectralConvVJPResidual alpha eR) $
+#endif
+                  \a -> let (x', h', y', t') = A.unlift a
+                                                :: ( Acc (Array DIM3 (Complex e))
+                                                   , Acc (Array DIM2 (Complex e))
+                                                   , Acc (Array DIM3 (Complex e))
+                                                   , Acc (Array DIM3 (Complex e)) )
+                             k = A.constant (P.realToFrac alpha :+ 0)
+                         in  convolveVJP x' h' (A.zipWith (\u v -> k * (u - v)) y' t')
+    in
+    go (A.lift (x, h, y, t))
 
 
 -- Pure definitions

@@ -49,6 +49,7 @@ module Data.Array.Accelerate.Math.FFT.LLVM.HIP (
   fft3DVJP,
   spectralConv,
   spectralConvVJP,
+  spectralConvVJPResidual,
 
 ) where
 
@@ -203,6 +204,42 @@ spectralConvVJP eR = ForeignAcc "hip.spectral.conv.vjp" go
               afft_spectral_conv_vjp (dtypeCode eR) (i64 b) (i64 h) (i64 w)
                                      (devicePtr d_x) (devicePtr d_h) (devicePtr d_gy)
                                      (devicePtr d_gx) (devicePtr d_gH) s
+      put future (((), agx), agH)
+      return future
+
+-- | The adjoint of 'spectralConv' for the squared-error loss L = (alpha/2) sum |y - t|^2: given
+-- x, H, y and the target t, the pair of 'spectralConvVJP' for gy = alpha (y - t), which the first
+-- row pass forms as it loads y and t (gy is never stored).
+spectralConvVJPResidual
+    :: Double
+    -> NumericR e
+    -> ForeignAcc ((((((), Array DIM3 (Vec2 e)), Array DIM2 (Vec2 e)), Array DIM3 (Vec2 e)), Array DIM3 (Vec2 e))
+                   -> (((), Array DIM3 (Vec2 e)), Array DIM2 (Vec2 e)))
+spectralConvVJPResidual alpha eR = ForeignAcc "hip.spectral.conv.vjp.residual" go
+  where
+    go (((((), ax), ah), ay), at) = do
+      let sh@((((), b), h), w) = shape ax
+          shH                  = shape ah
+      requireFilter "spectralConvVJPResidual" shH h w
+      if shape ay P.== sh P.&& shape at P.== sh then return ()
+        else internalError "spectralConvVJPResidual: y and t must have the shape of x"
+      agx    <- allocateRemote (arrayR dim3 eR) sh
+      agH    <- allocateRemote (arrayR dim2 eR) shH
+      stream <- asks hipStream
+      future <- new
+      liftPar $
+        withArray eR ax  stream $ \d_x  ->
+        withArray eR ah  stream $ \d_h  ->
+        withArray eR ay  stream $ \d_y  ->
+        withArray eR at  stream $ \d_t  ->
+        withArray eR agx stream $ \d_gx ->
+        withArray eR agH stream $ \d_gH ->
+          liftIO $ withStreamPtr stream $ \s ->
+            check "afft_spectral_conv_vjp_residual" =<<
+              afft_spectral_conv_vjp_residual (dtypeCode eR) (i64 b) (i64 h) (i64 w)
+                                              (devicePtr d_x) 0 (devicePtr d_h) (devicePtr d_y)
+                                              (devicePtr d_t) (realToFrac alpha)
+                                              (devicePtr d_gx) (devicePtr d_gH) s
       put future (((), agx), agH)
       return future
 

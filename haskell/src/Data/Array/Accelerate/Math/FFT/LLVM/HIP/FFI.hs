@@ -36,6 +36,7 @@ module Data.Array.Accelerate.Math.FFT.LLVM.HIP.FFI (
   afft_fft_vjp,
   afft_spectral_conv_fwd,
   afft_spectral_conv_vjp,
+  afft_spectral_conv_vjp_residual,
 
   -- * Codes
   modeForward, modeReverse, modeInverse,
@@ -46,7 +47,7 @@ module Data.Array.Accelerate.Math.FFT.LLVM.HIP.FFI (
 
 import Data.Int                                                     ( Int64 )
 import Foreign.C.String                                             ( CString )
-import Foreign.C.Types                                              ( CInt(..) )
+import Foreign.C.Types                                              ( CDouble(..), CInt(..) )
 import Foreign.Ptr                                                  ( Ptr )
 
 
@@ -115,3 +116,10 @@ foreign import ccall safe "afft_spectral_conv_fwd"
 --                            void *stream)
 foreign import ccall safe "afft_spectral_conv_vjp"
   afft_spectral_conv_vjp :: CInt -> Int64 -> Int64 -> Int64 -> Ptr () -> Ptr () -> Ptr () -> Ptr () -> Ptr () -> Ptr () -> IO CInt
+
+-- int afft_spectral_conv_vjp_residual(int dtype, int64_t batch, int64_t h, int64_t w, const void *d_x,
+--                                     int x_is_spectrum, const void *d_H, const void *d_y,
+--                                     const void *d_t, double alpha, void *d_gx, void *d_gH,
+--                                     void *stream)
+foreign import ccall safe "afft_spectral_conv_vjp_residual"
+  afft_spectral_conv_vjp_residual :: CInt -> Int64 -> Int64 -> Int64 -> Ptr () -> CInt -> Ptr () -> Ptr () -> Ptr () -> CDouble -> Ptr () -> Ptr () -> Ptr () -> IO CInt
