@@ -133,22 +133,18 @@ afft_rows(const RowArgs a)
     auto load_tile = [&](int64_t tile) {
         if (tile < ntiles && tile * C + c < nrows) {
             const int64_t row = src_row(tile * C + c);
-            const int64_t ro = (roffs ? roffs[row] : row * N) + t;
-            const V *p = in + ro;
+            const V *p = in + (roffs ? roffs[row] : row * N) + t;
             if (nt_in) {
 #pragma unroll
                 for (int i = 0; i < E; ++i) nxt[i] = c_load_nt(p + i * T);
-                if constexpr (RES) {
-#pragma unroll
-                    for (int i = 0; i < E; ++i) nxt2[i] = c_load_nt(in2 + ro + i * T);
-                }
             } else {
 #pragma unroll
                 for (int i = 0; i < E; ++i) nxt[i] = p[i * T];
-                if constexpr (RES) {
+            }
+            if constexpr (RES) {
+                const V *q = in2 + (roffs ? roffs[row] : row * N) + t;
 #pragma unroll
-                    for (int i = 0; i < E; ++i) nxt2[i] = in2[ro + i * T];
-                }
+                for (int i = 0; i < E; ++i) nxt2[i] = nt_in ? c_load_nt(q + i * T) : q[i * T];
             }
         }
     };
@@ -190,14 +186,18 @@ afft_rows(const RowArgs a)
         int tl = t;
         if constexpr (LD) __asm__ volatile("" : "+v"(tl));
         V reg[E];
+        if constexpr (RES) {
+            const Real al = (Real)a.alpha;
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            V v = nxt[i];
-            if constexpr (RES) {
-                const Real al = (Real)a.alpha;
-                v = c_make<V>(al * (v.x - nxt2[i].x), al * (v.y - nxt2[i].y));
+            for (int i = 0; i < E; ++i) {
+                const V v = c_make<V>(al * (nxt[i].x - nxt2[i].x), al * (nxt[i].y - nxt2[i].y));
+                reg[i] = cin ? c_conj(v) : v;
             }
-            reg[i] = cin ? c_conj(v) : v;
+        } else {
+            // (this exact form: written through a temporary, the compiler merged the streaming
+            // and the plain load paths and dropped the streaming hint, 7% on the 4096-point rows)
+#pragma unroll
+            for (int i = 0; i < E; ++i) reg[i] = cin ? c_conj(nxt[i]) : nxt[i];
         }
         if (a.sg_in.on && gt * C + c < nrows) {
             const int64_t r0 = src_row(gt * C + c);

@@ -33,6 +33,7 @@ One step = one call.
   --profile-headline runs only the headline plan (and a plain device copy of the same array,
              the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE) for profiling.
   --profile-workload W  runs only workload W (fft2d, fft3d, four_step, ragged, spectral,
+             spectral_residual,
              batched2d) for profiling.
              Both first run untimed calls for 0.3 s (settle_calls in their output line), then
              warmup + steps calls: a profile's steady state is its last `steps` launches.
@@ -300,8 +301,8 @@ def ragged_extras(afft, Mode, torch, dev, args, chk):
     flops = sum(5.0 * n * math.log2(n) for n in lengths)
     res = {"ragged_total_elements": total,
            "ragged_autotune": {"status": int(st), "seconds": round(tune_s, 2)}}
-    t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(5, args.steps // 2), 3)
-    tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(5, args.steps // 2), 3)
+    t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(10, args.steps), 3)
+    tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(10, args.steps), 3)
     got = out.cpu().numpy()
     # plain: one fft1D call per transform, straight through the C ABI (no tensor allocation)
     offs = [0]
@@ -402,10 +403,10 @@ def spectral_extras(afft, Mode, torch, dev, args, chk):
             gH += torch.conj(Fx) * Fg
         return torch.stack(gxs), gH * (1.0 / M)
 
-    tf = time_calls(fused, max(3, args.steps // 10), 2)
-    tfs = time_calls(fused_saved, max(3, args.steps // 10), 2)
-    tfr = time_calls(fused_residual, max(3, args.steps // 10), 2)
-    tfrs = time_calls(fused_residual_saved, max(3, args.steps // 10), 2)
+    tf = time_calls(fused, max(10, args.steps // 2), 3)
+    tfs = time_calls(fused_saved, max(10, args.steps // 2), 3)
+    tfr = time_calls(fused_residual, max(10, args.steps // 2), 3)
+    tfrs = time_calls(fused_residual_saved, max(10, args.steps // 2), 3)
     tp = time_calls(plain, 3, 1)
     tfwd = time_calls(lambda: afft.spectral_conv(x, H), max(3, args.steps // 5), 2)
     res = {"spectral_conv_fused_fwd_vjp": {"us": round(tf * 1e6, 1)},
@@ -472,8 +473,20 @@ def _prof_spectral(afft, Mode, torch, dev):
     return call
 
 
+def _prof_spectral_residual(afft, Mode, torch, dev):
+    x = to_device(splitmix_normal(5, (64, 512, 512), "complex64"), dev)
+    H = to_device(splitmix_normal(6, (512, 512), "complex64"), dev)
+    t = to_device(splitmix_normal(7, (64, 512, 512), "complex64"), dev)
+
+    def call():
+        y = afft.spectral_conv(x, H)
+        afft.spectral_conv_vjp_residual(x, H, y, t, 2.0)
+    return call
+
+
 PROFILE_WORKLOADS = {"fft2d": _prof_fft2d, "fft3d": _prof_fft3d, "four_step": _prof_four_step,
-                     "batched2d": _prof_batched2d, "ragged": _prof_ragged, "spectral": _prof_spectral}
+                     "batched2d": _prof_batched2d, "ragged": _prof_ragged, "spectral": _prof_spectral,
+                     "spectral_residual": _prof_spectral_residual}
 
 
 def main():
@@ -507,14 +520,26 @@ def main():
         torch.cuda.synchronize()
         return n
 
+    def profiled(call, steps, warmup):
+        """warmup calls, then `steps` calls between HIP events on the current stream: the
+        average the profile's steady state is compared with"""
+        for _ in range(warmup):
+            call()
+        s = torch.cuda.current_stream()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(steps):
+            call()
+        b.record(s)
+        torch.cuda.synchronize()
+        return round(a.elapsed_time(b) * 1e3 / max(1, steps), 2)
+
     if args.profile_workload:
         call = PROFILE_WORKLOADS[args.profile_workload](afft, Mode, torch, dev)
         ns = settle(call)
-        for _ in range(args.warmup + args.steps):
-            call()
-        torch.cuda.synchronize()
-        print(json.dumps({"profile_workload": args.profile_workload, "settle_calls": ns, "steps": args.steps}),
-              flush=True)
+        us = profiled(call, args.steps, args.warmup)
+        print(json.dumps({"profile_workload": args.profile_workload, "settle_calls": ns, "steps": args.steps,
+                          "us_per_call": us}), flush=True)
         return
 
     rows, n = 16384, 4096
@@ -529,12 +554,12 @@ def main():
 
     if args.profile_headline:
         ns = settle(step)
-        for _ in range(args.warmup + args.steps):
-            step()
+        us = profiled(step, args.steps, args.warmup)
         for _ in range(args.steps):
             out.copy_(x)  # plain copy of the same 512 MiB: counter calibration
         torch.cuda.synchronize()
-        print(json.dumps({"profile_headline": True, "settle_calls": ns, "steps": args.steps}), flush=True)
+        print(json.dumps({"profile_headline": True, "settle_calls": ns, "steps": args.steps, "us_per_call": us}),
+              flush=True)
         return
 
     # settle: untimed steps until 50 ms have passed, so that the clocks of a freshly started
