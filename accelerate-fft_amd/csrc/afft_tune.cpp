@@ -8,12 +8,14 @@
 //   strided  a single-pass strided tile: elements per thread E (which fixes the radices and their
 //            order) and columns per workgroup C, encoded C * 1000 + E
 //   regcol   lanes per column of the register-resident kernel (1, or 2 / 4 through DPP)
-//   ragged   the cap on elements per thread of a class of the ragged persistent kernel (which
-//            fixes its radices and the transforms per work item)
+//   ragged   the layout of a class of the ragged persistent kernel: 0 for its compiled layout,
+//            else the cap on elements per thread of the pass-table layout (which fixes its radices
+//            and the transforms per work item)
 // afft_autotune records the decision points of the plan it would build, then, one point after the
 // other, builds the whole plan with each candidate forced (the choices made so far kept), times
 // it on the device with HIP events, and records the fastest; afft_ragged_autotune does the same
-// per length class of a ragged batch, timing each class alone.  Plans built afterwards use the
+// per length class of a ragged batch, timing each class alone (a class of the persistent kernel
+// on at least 2^23 elements).  Plans built afterwards use the
 // choices.  The table is keyed by (gfx arch, HIP runtime version, kind, dtype, length) and
 // persisted as text, one entry per line:
 //     afft-tune 1
