@@ -63,7 +63,7 @@ def test_fft2D_many_64x512x512(gpu):
 
 def test_spectral_configuration_all_64_images(gpu):
     # y = Inverse(fft2D(x) * H), loss = sum |y - t|^2: x 64 x 512 x 512, H 512 x 512, t, all
-    # normal (seeds 5, 6, 7); the gradients from gy = 2 (y - t), both VJP forms
+    # normal (seeds 5, 6, 7); the gradients from gy = 2 (y - t), every VJP form
     B, h, w = 64, 512, 512
     xh = O.normal(5, (B, h, w), np.complex64)
     Hh = O.normal(6, (h, w), np.complex64)
@@ -74,6 +74,9 @@ def test_spectral_configuration_all_64_images(gpu):
     gy = 2 * (y - t)
     gx, gH = afft.spectral_conv_vjp(x, H, gy)
     sx, sH = afft.spectral_conv_vjp(None, H, gy, spectrum=spec)
+    # the squared-error form (gy = 2 (y - t) formed on load), from x and from the spectrum
+    rx, rH = afft.spectral_conv_vjp_residual(x, H, y, t, 2.0)
+    qx, qH = afft.spectral_conv_vjp_residual(None, H, y, t, 2.0, spectrum=spec)
     Fx = np.stack([O.fft2D(O.Mode.Forward, xb) for xb in xh])
     assert O.rel_l2(spec.cpu().numpy(), Fx) <= tol
     wy = np.stack([O.fft2D(O.Mode.Inverse, f * Hh) for f in Fx])
@@ -82,7 +85,7 @@ def test_spectral_configuration_all_64_images(gpu):
     Fg = np.stack([O.fft2D(O.Mode.Forward, gb) for gb in g])
     wgx = np.stack([O.fft2D(O.Mode.Reverse, np.conj(Hh) * f) / (h * w) for f in Fg])
     wgH = np.sum(np.conj(Fx) * Fg, axis=0) / (h * w)
-    for got_x, got_H in ((gx, gH), (sx, sH)):
+    for got_x, got_H in ((gx, gH), (sx, sH), (rx, rH), (qx, qH)):
         assert O.rel_l2(got_x.cpu().numpy(), wgx) <= tol
         assert O.rel_l2(got_H.cpu().numpy(), wgH) <= tol
     # image by image: every image of gx within the bound
