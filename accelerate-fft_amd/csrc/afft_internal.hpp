@@ -107,6 +107,10 @@ struct FftArgs {
     int post_shift;
     const void *post_hi;
     const void *post_lo;
+    // the same twiddles as one precomputed grid, W_L^{jl k} at post_tab[k * post_ld + jl]: one
+    // load per element, coalesced across the lanes of a strided tile (or null: the two-level table)
+    const void *post_tab;
+    int64_t post_ld;
     int xcd_remap;       // map workgroups to tiles XCD-contiguously (strided passes)
     int nt_in, nt_out;   // streaming loads / stores (plan input read once / final output)
     SignSpec sg_in, sg_out; // centring signs on load (first pass) / on store (last pass)
@@ -222,6 +226,8 @@ hipError_t launch_bluestein(int dtype, const BluesteinArgs &a, int block, int64_
 // FP64 twiddle generation, rounded to the element type.
 hipError_t gen_pass_twiddles(int dtype, void *out, const PassTable &pt, hipStream_t s);
 // out[i] = W_L^{i * step} for i < count (two-level four-step tables).
+// out[q * cols + j] = W_L^{q j}, q < rows, j < cols, rows * cols <= L (FP64, rounded)
+hipError_t gen_twiddle_grid(int dtype, void *out, int64_t L, int64_t rows, int64_t cols, hipStream_t s);
 hipError_t gen_power_table(int dtype, void *out, int64_t L, int64_t step, int64_t count,
                            hipStream_t s);
 // out[m] = exp(-i pi (m^2 mod 2n) / n) for m < n (chirp), in the element type.

@@ -44,10 +44,14 @@ __device__ __forceinline__ V *lds_slot(V *lds, int c, int C, int n, int interlea
 // Batched Stockham transform.  N = 0: the generic transform of any supported n, driven by the pass
 // table; N > 0: n = N with compile-time radices Rs (CtStockham, the row kernels' passes; used for
 // the interleaved tiles of strided axes), twiddle bases loaded once from the table W_N^m.
+// TWG: the four-step twiddles on store come from the precomputed grid post_tab, loaded with the
+// data so that they are in flight during the transform instead of in front of every store (the
+// middle pass of the split six-step); otherwise from the two-level table post_hi / post_lo.
 
-template <typename Real, int E, int N, int... Rs>
+template <typename Real, int E, int N, bool TWG, int... Rs>
 __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_stockham(const FftArgs a)
 {
+    static_assert(!TWG || N != 0, "the twiddle grid goes with the compile-time tiles");
     using V = typename cplx_of<Real>::type;
     extern __shared__ __align__(16) unsigned char afft_smem[];
     V *lds = reinterpret_cast<V *>(afft_smem);
@@ -80,6 +84,14 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
 #pragma unroll
         for (int i = 0; i < E; ++i) reg[i] = c_make<V>(0, 0);
     }
+    V twv[TWG ? E : 1];
+    if constexpr (TWG) {
+        if (active) {
+            const V *tab = reinterpret_cast<const V *>(a.post_tab) + addr_level(a.ain, j, a.post_level);
+#pragma unroll
+            for (int i = 0; i < E; ++i) twv[i] = tab[(int64_t)(t + i * T) * a.post_ld];
+        }
+    }
 
     int es;
     V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, n, a.lds_interleave, es);
@@ -99,7 +111,9 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         for (int i = 0; i < E; ++i) {
             const int k = t + i * T;
             V v = reg[i];
-            if (a.post_L) {
+            if constexpr (TWG) {
+                v = c_mul(v, twv[TWG ? i : 0]);
+            } else if (a.post_L) {
                 // jl < n2 and k < n1, so jl * k < n1 * n2 = post_L: no reduction needed
                 const int64_t e = jl * (int64_t)k;
                 v = c_mul(v, power_lookup(reinterpret_cast<const V *>(a.post_hi),
@@ -275,6 +289,20 @@ __global__ void afft_gen_power_table(typename cplx_of<Real>::type *out, int64_t 
 }
 
 template <typename Real>
+__global__ void afft_gen_twiddle_grid(typename cplx_of<Real>::type *out, int64_t L, int64_t rows, int64_t cols)
+{
+    using V = typename cplx_of<Real>::type;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows * cols;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = i / cols, j = i - q * cols;
+        const int64_t e = (q * j) % L;
+        double s, co;
+        sincospi(-2.0 * (double)e / (double)L, &s, &co);
+        out[i] = c_make<V>((Real)co, (Real)s);
+    }
+}
+
+template <typename Real>
 __global__ void afft_gen_chirp(typename cplx_of<Real>::type *out, int64_t n)
 {
     using V = typename cplx_of<Real>::type;
@@ -362,15 +390,23 @@ static hipError_t launch_stockham_t(const FftArgs &a, int block, int64_t grid, s
     // chooser picks for these lengths)
     if (a.pw && a.lds_interleave && a.E == 16) {
         if constexpr (sizeof(Real) == 4) {
-            if (a.pt.n == 128) return launch_one(afft_stockham<Real, 16, 128, 16, 8>, a, block, grid, lds, s);
-            if (a.pt.n == 512) return launch_one(afft_stockham<Real, 16, 512, 16, 16, 2>, a, block, grid, lds, s);
-            if (a.pt.n == 1024) return launch_one(afft_stockham<Real, 16, 1024, 16, 16, 4>, a, block, grid, lds, s);
+            // the twiddle-grid forms in Complex Float only: in FP64 its registers would cost the
+            // strided tiles a wave per SIMD
+            if (a.post_tab) {
+                if (a.pt.n == 128) return launch_one(afft_stockham<Real, 16, 128, true, 16, 8>, a, block, grid, lds, s);
+                if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, true, 16, 16>, a, block, grid, lds, s);
+                if (a.pt.n == 512) return launch_one(afft_stockham<Real, 16, 512, true, 16, 16, 2>, a, block, grid, lds, s);
+                if (a.pt.n == 1024) return launch_one(afft_stockham<Real, 16, 1024, true, 16, 16, 4>, a, block, grid, lds, s);
+            }
+            if (a.pt.n == 128) return launch_one(afft_stockham<Real, 16, 128, false, 16, 8>, a, block, grid, lds, s);
+            if (a.pt.n == 512) return launch_one(afft_stockham<Real, 16, 512, false, 16, 16, 2>, a, block, grid, lds, s);
+            if (a.pt.n == 1024) return launch_one(afft_stockham<Real, 16, 1024, false, 16, 16, 4>, a, block, grid, lds, s);
         }
-        if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, 16, 16>, a, block, grid, lds, s);
+        if (a.pt.n == 256) return launch_one(afft_stockham<Real, 16, 256, false, 16, 16>, a, block, grid, lds, s);
     }
     switch (a.E) {
 #define X(EV)                                                                                    \
-    case EV: return launch_one(afft_stockham<Real, EV, 0>, a, block, grid, lds, s);
+    case EV: return launch_one(afft_stockham<Real, EV, 0, false>, a, block, grid, lds, s);
         AFFT_FOR_EACH_E(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -438,6 +474,16 @@ hipError_t gen_power_table(int dtype, void *out, int64_t L, int64_t step, int64_
         hipLaunchKernelGGL(afft_gen_power_table<float>, dim3(g), dim3(256), 0, s, (float2 *)out, L, step, count);
     else
         hipLaunchKernelGGL(afft_gen_power_table<double>, dim3(g), dim3(256), 0, s, (double2 *)out, L, step, count);
+    return hipGetLastError();
+}
+
+hipError_t gen_twiddle_grid(int dtype, void *out, int64_t L, int64_t rows, int64_t cols, hipStream_t s)
+{
+    const int64_t g = gs_grid(rows * cols);
+    if (dtype == 0)
+        hipLaunchKernelGGL(afft_gen_twiddle_grid<float>, dim3(g), dim3(256), 0, s, (float2 *)out, L, rows, cols);
+    else
+        hipLaunchKernelGGL(afft_gen_twiddle_grid<double>, dim3(g), dim3(256), 0, s, (double2 *)out, L, rows, cols);
     return hipGetLastError();
 }
 

@@ -333,7 +333,8 @@ struct Builder {
     // Plain single-workgroup transform (n 5-smooth and within the LDS limit).
     void add_stockham(int64_t n, int64_t ntrans, const Addr &ain, const Addr &aout, bool strided,
                       Slot src, Slot dst, int64_t post_L = 0, int post_level = 0,
-                      const void *post_hi = nullptr, const void *post_lo = nullptr, int post_shift = 0)
+                      const void *post_hi = nullptr, const void *post_lo = nullptr, int post_shift = 0,
+                      const void *post_tab = nullptr, int64_t post_ld = 0)
     {
         if (strided && regcol_supported(plan.dtype, n) && lanes_adjacent(ain) && lanes_adjacent(aout) &&
             !getenv_flag("AFFT_NO_REGCOL")) {
@@ -401,6 +402,8 @@ struct Builder {
         a.post_hi = post_hi;
         a.post_lo = post_lo;
         a.post_shift = post_shift;
+        a.post_tab = post_tab;
+        a.post_ld = post_ld;
         // strided tiles of 128 ... 1024 (Complex Float) and 256 (Complex Double) run compile-time
         // passes (the launcher picks them when the power table is present)
         const bool ct = plan.dtype == AFFT_C64 ? (n == 128 || n == 256 || n == 512 || n == 1024) : n == 256;
@@ -791,7 +794,14 @@ struct Builder {
         ja.st1 = n2;
         ja.st2 = n;
         ja.es = m2;
-        add_stockham(m1, o * n1 * m2, ja, ja, true, cur, cur, n2, 0, hi2, lo2, sh2);
+        // the twiddles W_n2^{j2 q1} as one m1 x m2 grid (n2 elements): per element one load,
+        // coalesced across a tile's lanes, where the two-level table took two scattered loads
+        // (measured: those lookups cost 5 of the 47 us of a 2^22 row)
+        void *grid = alloc(n2 * (int64_t)esz());
+        if (status) return;
+        check(gen_twiddle_grid(plan.dtype, grid, n2, m1, m2, s));
+        if (status) return;
+        add_stockham(m1, o * n1 * m2, ja, ja, true, cur, cur, n2, 0, hi2, lo2, sh2, grid, m2);
         if (status) return;
         // row r = k1 + n1*(q1 + m1*b) of transform b starts at b*n + k1*n2 + q1*m2
         const int64_t nrows = o * n1 * m1;
