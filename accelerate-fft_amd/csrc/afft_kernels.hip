@@ -171,6 +171,15 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
         for (int i = 0; i < E; ++i) reg[i] = c_make<V>(0, 0);
     }
 
+    // the filter is read with the data when the registers allow it (the 512-thread layout of
+    // 10240 keeps two waves per SIMD with it), so that its loads are in flight during the first
+    // transform instead of in front of the second
+    constexpr bool kEarlyFilt = MC != 0 && E <= 20;
+    V fv[kEarlyFilt ? E : 1];
+    if constexpr (kEarlyFilt) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) fv[i] = filt[t + i * T];
+    }
     int es;
     V *slot = lds_slot(lds, c < a.C ? c : 0, a.C, M, a.lds_interleave, es);
     if constexpr (MC != 0) {
@@ -183,8 +192,13 @@ __global__ void __launch_bounds__(max_block_threads(sizeof(Real) == 8, E)) afft_
     } else {
         fft_core<E>(reg, slot, es, a.pt, tw, t, T);
     }
+    if constexpr (kEarlyFilt) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
+        for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], fv[kEarlyFilt ? i : 0]));
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
+    }
     if constexpr (MC != 0) {
         int tl = t;
         __asm__ volatile("" : "+v"(tl));
