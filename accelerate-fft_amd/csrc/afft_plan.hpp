@@ -122,7 +122,7 @@ bool is_5smooth(int64_t n);
 // The autotuner (afft_tune.cpp).  The planner asks tune_choice for every layout decision it
 // makes for a length, with the candidates it accepts (the first is its default); the answer is a
 // choice forced by the autotuner on this thread, else the tuned one from the table, else -1.
-enum TuneKind { kTuneRows = 0, kTuneStrided = 1, kTuneRegcol = 2, kTuneRagged = 3, kTuneKinds = 4 };
+enum TuneKind { kTuneRows = 0, kTuneStrided = 1, kTuneRegcol = 2, kTuneRagged = 3, kTuneRaggedSplit = 4, kTuneKinds = 5 };
 struct TunePoint {
     int kind, dtype;
     int64_t n;

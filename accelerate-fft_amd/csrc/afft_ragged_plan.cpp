@@ -32,6 +32,7 @@ struct RaggedLong {
     Plan plan;
     int64_t L = 0;
     const int64_t *rows = nullptr; // device: element offsets of the class's rows
+    int64_t elems = 0;             // count * L
     bool small = false;            // too few elements to fill the device: runs on the side stream
 };
 
@@ -40,7 +41,8 @@ struct RaggedLong {
 // ones, and the threshold balances the two streams (measured on the 10,000-transform batch:
 // 1M -> 8M moved the 5120...12288 classes to the side stream, 550 -> 465 us; once the large
 // classes took fewer passes, 8M -> 4M moved 12288 x 432 back, 392 -> 368 us, while 2.8M, which
-// also moves 8192 x 357, gave 400 us).
+// also moves 8192 x 357, gave 400 us).  This is the default; afft_ragged_autotune times every
+// partition of a batch's long classes by size and keeps the best for that batch.
 constexpr int64_t kSmallLongElems = 1 << 22;
 
 // The per-execution resources of a ragged batch: a side stream forked from and joined back to
@@ -221,6 +223,24 @@ template <typename T> T *upload(RaggedPlan &rp, const DeviceInfo &dev, const std
 
 } // namespace
 
+// A key for a ragged batch's length classes (FNV-1a over (length, count) in length order), the
+// autotuner's `length` for batch-wide decisions; 62 bits, so it stays a positive int64
+static int64_t ragged_batch_key(const std::map<int64_t, std::vector<int>> &by_len)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v >> (8 * b)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    for (const auto &kv : by_len) {
+        mix((uint64_t)kv.first);
+        mix((uint64_t)kv.second.size());
+    }
+    return (int64_t)(h >> 2);
+}
+
 static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo &dev, hipStream_t s)
 {
     rp.tables.dtype = rp.dtype;
@@ -254,11 +274,7 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             if (b.status) return b.status;
             auto lp = std::make_unique<RaggedLong>();
             lp->L = kv.first;
-            static const int64_t small_elems = [] {
-                const char *v = std::getenv("AFFT_RAGGED_SMALL");
-                return v ? (int64_t)std::atoll(v) : kSmallLongElems;
-            }();
-            lp->small = (int64_t)cnt * kv.first < small_elems;
+            lp->elems = (int64_t)cnt * kv.first;
             const int64_t ext[2] = {cnt, kv.first};
             // rows packed back to back start at arbitrary 8- or 16-byte offsets
             lp->plan.flags = kPlanUnalignedRows;
@@ -276,6 +292,30 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
             const int c = std::min(K.C, cnt - j);
             items.push_back({{cls, first + j, c}, per * c});
         }
+    }
+    // which long classes go to the side stream: those below a size in elements, the default or
+    // the one tuned for this batch (its classes hashed), or AFFT_RAGGED_SMALL
+    {
+        const char *env = std::getenv("AFFT_RAGGED_SMALL");
+        int64_t small_elems = env ? (int64_t)std::atoll(env) : kSmallLongElems;
+        if (!env && !rp.longs.empty()) {
+            std::vector<int64_t> sizes;
+            for (const auto &lp : rp.longs) sizes.push_back(lp->elems);
+            std::sort(sizes.begin(), sizes.end());
+            // the default, then every other partition: none aside, the smallest 1, 2, ... aside
+            auto aside = [&](int64_t t) { return std::lower_bound(sizes.begin(), sizes.end(), t) - sizes.begin(); };
+            std::vector<int> cands = {(int)std::min<int64_t>(kSmallLongElems, INT32_MAX)};
+            std::vector<int64_t> seen = {aside(cands[0])};
+            for (size_t k = 0; k <= sizes.size(); ++k) {
+                const int64_t t = std::min<int64_t>(k ? sizes[k - 1] + 1 : 0, INT32_MAX);
+                if (std::find(seen.begin(), seen.end(), aside(t)) != seen.end()) continue;
+                seen.push_back(aside(t));
+                cands.push_back((int)t);
+            }
+            const int t = tune_choice(kTuneRaggedSplit, rp.dtype, ragged_batch_key(by_len), rp.device, cands);
+            if (t >= 0) small_elems = t;
+        }
+        for (auto &lp : rp.longs) lp->small = lp->elems < small_elems;
     }
     std::stable_sort(items.begin(), items.end(), [](const Tmp &x, const Tmp &y) { return x.cost > y.cost; });
     std::vector<RaggedItem> its;
@@ -534,7 +574,8 @@ extern "C" int afft_ragged_plan_destroy(afft_ragged_plan *plan)
 // of one length, its decision points (the persistent kernel's layout, or the layouts inside the
 // batched plan of a long class) timed candidate by candidate.  A class of the persistent kernel
 // is timed on at least 2^23 elements (its length repeated): a few hundred short transforms take
-// less device time than the host takes to issue the call, which would hide the layout.
+// less device time than the host takes to issue the call, which would hide the layout.  Then,
+// with those layouts, the whole batch: which of its long classes run on the side stream.
 extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *lengths, int device)
 {
     if (dtype != AFFT_C64 && dtype != AFFT_C128) return AFFT_ERROR_UNSUPPORTED_DTYPE;
@@ -558,6 +599,9 @@ extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *len
         tune_record(nullptr);
         if (st) break;
         (void)afft_ragged_plan_destroy(p);
+        points.erase(std::remove_if(points.begin(), points.end(),
+                                    [](const TunePoint &q) { return q.kind == kTuneRaggedSplit; }),
+                     points.end());
         if (points.empty()) continue;
         const bool persistent = std::all_of(points.begin(), points.end(),
                                             [](const TunePoint &q) { return q.kind == kTuneRagged; });
@@ -580,6 +624,42 @@ extern "C" int afft_ragged_autotune(int dtype, int64_t count, const int64_t *len
         });
         (void)hipFree(buf);
         if (st) break;
+    }
+    if (!st && count > 0) {
+        std::vector<TunePoint> points;
+        afft_ragged_plan *p = nullptr;
+        tune_record(&points);
+        st = afft_ragged_plan_create(&p, dtype, count, lengths, device);
+        tune_record(nullptr);
+        if (!st) (void)afft_ragged_plan_destroy(p);
+        points.erase(std::remove_if(points.begin(), points.end(),
+                                    [](const TunePoint &q) { return q.kind != kTuneRaggedSplit; }),
+                     points.end());
+        if (!st && !points.empty()) {
+            int64_t total = 0;
+            for (auto &kv : by_len) total += kv.first * kv.second;
+            // timed as replays of the batch's graph (the form whose launch gaps are smallest, so
+            // the balance of the two streams shows most), on a stream of its own
+            void *buf = nullptr;
+            hipStream_t ts = nullptr;
+            const size_t bytes = (size_t)total * esz;
+            if (hipMalloc(&buf, bytes) != hipSuccess || hipMemset(buf, 0, bytes) != hipSuccess) {
+                st = AFFT_ERROR_ALLOC;
+            } else if (hipStreamCreate(&ts) != hipSuccess) {
+                st = AFFT_ERROR_HIP;
+            } else {
+                st = tune_points(points, device, [&]() -> double {
+                    afft_ragged_plan *q = nullptr;
+                    if (afft_ragged_plan_create(&q, dtype, count, lengths, device)) return -1;
+                    const double us = tune_time_calls(
+                        [&](hipStream_t s) { return afft_ragged_exec_graph(q, AFFT_FORWARD, buf, buf, s); }, ts);
+                    (void)afft_ragged_plan_destroy(q);
+                    return us;
+                });
+            }
+            if (ts) (void)hipStreamDestroy(ts);
+            if (buf) (void)hipFree(buf);
+        }
     }
     if (old >= 0) (void)hipSetDevice(old);
     return st;

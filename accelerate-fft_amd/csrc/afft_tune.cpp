@@ -11,6 +11,8 @@
 //   ragged   the layout of a class of the ragged persistent kernel: 0 for its compiled layout,
 //            else the cap on elements per thread of the pass-table layout (which fixes its radices
 //            and the transforms per work item)
+//   ragged_split  for a whole ragged batch (keyed by a hash of its length classes): the size in
+//            elements below which a long class runs on the side stream
 // afft_autotune records the decision points of the plan it would build, then, one point after the
 // other, builds the whole plan with each candidate forced (the choices made so far kept), times
 // it on the device with HIP events, and records the fastest; afft_ragged_autotune does the same
@@ -46,7 +48,7 @@ namespace afft {
 
 namespace {
 
-const char *const kKindName[kTuneKinds] = {"rows", "strided", "regcol", "ragged"};
+const char *const kKindName[kTuneKinds] = {"rows", "strided", "regcol", "ragged", "ragged_split"};
 
 struct TuneEntry {
     int value;
@@ -114,6 +116,7 @@ bool plausible(int kind, int dtype, int64_t n, int value)
     }
     case kTuneRegcol: return (value == 1 || value == 2 || value == 4) && regcol_supported(dtype, n);
     case kTuneRagged: return value >= 0 && value <= 32; // 0: the compiled layout
+    case kTuneRaggedSplit: return value >= 0;          // elements below which a class goes aside
     default: return false;
     }
 }

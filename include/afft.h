@@ -269,9 +269,10 @@ int afft_spectral_conv_vjp_residual(int dtype, int64_t batch, int64_t h, int64_t
  * cache is cleared).  afft_ragged_autotune does the same for every length class of a ragged batch,
  * each timed alone (for a class of the persistent kernel its compiled layout against the
  * pass-table layouts of each cap on elements per thread, timed on at least 2^23 elements; the
- * layouts of the long classes' plans).  afft_tune_save/afft_tune_load persist the choices as a text file keyed by
- * (gfx arch, HIP runtime version, decision kind, dtype, length); stale or malformed lines are
- * ignored. */
+ * layouts of the long classes' plans), then for the whole batch which long classes run on the
+ * side stream (keyed by a hash of the batch's length classes).  afft_tune_save/afft_tune_load
+ * persist the choices as a text file keyed by (gfx arch, HIP runtime version, decision kind,
+ * dtype, length); stale or malformed lines are ignored. */
 int afft_autotune(int dtype, int rank, const int64_t *extents, int axes, int device);
 int afft_ragged_autotune(int dtype, int64_t count, const int64_t *lengths, int device);
 int afft_tune_save(const char *path);

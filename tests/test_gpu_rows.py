@@ -57,11 +57,15 @@ def test_autotune_round_trip(gpu, tmp_path):
     assert lines[0] == "afft-tune 1"
     rows = [l.split() for l in lines[1:]]
     kinds = {r[2] for r in rows}
-    assert {"rows", "strided", "regcol", "ragged"} <= kinds, kinds
+    assert {"rows", "strided", "regcol", "ragged", "ragged_split"} <= kinds, kinds
     assert any(r[2:5] == ["rows", "0", "4096"] for r in rows)
     assert any(r[2:5] == ["strided", "0", "512"] for r in rows)
     assert any(r[2:5] == ["regcol", "0", "64"] for r in rows)
     assert any(r[2] == "ragged" and r[4] in ("64", "1000", "4096", "257") for r in rows)
+    # the whole batch's side-stream split: the default (its one long class, 20480 x 8, aside) or
+    # none aside
+    split = [r for r in rows if r[2] == "ragged_split"]
+    assert len(split) == 1 and int(split[0][5]) in (0, 1 << 22), split
     assert all(r[0].startswith("gfx") for r in rows)
     # a stale and a corrupt entry next to the good ones are ignored on load
     arch, hv = rows[0][0], rows[0][1]
