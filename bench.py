@@ -359,6 +359,52 @@ def ragged_extras(afft, Mode, torch, dev, args, chk):
     return res
 
 
+def ragged_large_extras(afft, Mode, torch, dev, args, chk):
+    """The same length distribution at about 2^27 elements (the first transforms of a longer draw
+    whose total stays within 2^27, about 25,000 of them): one autotuned ragged plan, eager and
+    replayed, against one batched call per length on batches gathered beforehand."""
+    import ctypes
+    import numpy as np
+    draw = ragged_lengths(count=40000, seed=8)
+    cum = np.cumsum(draw)
+    lengths = draw[:int(np.searchsorted(cum, 1 << 27, side="right"))]
+    total = sum(lengths)
+    lib = afft._lib.lib()
+    st = lib.afft_ragged_autotune(0, len(lengths), (ctypes.c_int64 * len(lengths))(*lengths), 0)
+    xh = splitmix_uniform(9, (total,), "complex64")
+    x = to_device(xh, dev)
+    out = torch.empty_like(x)
+    plan = afft.RaggedPlan(lengths, torch.complex64)
+    flops = sum(5.0 * n * math.log2(n) for n in lengths)
+    t = time_calls(lambda: plan.execute(Mode.Forward, x, out=out), max(10, args.steps // 2), 3)
+    tg = time_calls(lambda: plan.execute(Mode.Forward, x, out=out, graph=True), max(10, args.steps // 2), 3)
+    got = out.cpu().numpy()
+    offs = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    by = {}
+    for n, o in zip(lengths, offs):
+        by.setdefault(n, []).append(int(o))
+    groups = [(n, torch.stack([x[o:o + n] for o in os_])) for n, os_ in by.items()]
+
+    def per_length():
+        for n, g in groups:
+            afft.fft(Mode.Forward, g)
+
+    tl = time_calls(per_length, max(3, args.steps // 5), 1)
+    del groups
+    best = min(t, tg)
+    res = {"ragged_2p27": {"transforms": len(lengths), "total_elements": total, "autotune_status": int(st),
+                           "us": round(t * 1e6, 1), "graph_us": round(tg * 1e6, 1),
+                           "gflops": round(flops / best / 1e9, 1),
+                           "hbm_frac": round(16.0 * total / best / 1e9 / HBM_PEAK_GBS, 4),
+                           "per_length_batched_us": round(tl * 1e6, 1),
+                           "speedup_vs_per_length": round(tl / best, 2)}}
+    res["ragged_2p27"].update(chk.ragged(got, xh, lengths, Mode.Forward))
+    plan.close()
+    del x, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def spectral_extras(afft, Mode, torch, dev, args, chk):
     """y = Inverse(fft2D(x) * H) over 64 x 512 x 512 Complex Float, forward + VJP: the fused
     kernels (the VJP recomputing F x, and the forward storing it for the VJP) against the plain
@@ -665,6 +711,7 @@ def main():
         del c3
         torch.cuda.empty_cache()
         extras.update(ragged_extras(afft, Mode, torch, dev, args, chk))
+        extras.update(ragged_large_extras(afft, Mode, torch, dev, args, chk))
         extras.update(spectral_extras(afft, Mode, torch, dev, args, chk))
         line["extras"] = extras
 
