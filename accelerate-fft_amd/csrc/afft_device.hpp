@@ -72,6 +72,31 @@ template <typename V> __device__ __forceinline__ void c_store_nt(V *p, V a)
     __builtin_nontemporal_store(v, reinterpret_cast<vec2 *>(p));
 }
 template <typename V> __device__ __forceinline__ V c_load(const V *p, bool nt) { return nt ? c_load_nt(p) : *p; }
+
+// Pointers known to address global memory: loads and stores through them compile to global_*
+// instructions, not the flat ones a generic pointer gives inside a non-inlined function (flat
+// accesses also count against the LDS wait counter, so every LDS wait and barrier after them
+// would wait for them too).  The element moves as a builtin two-lane vector.
+template <typename V> using GIn = const V __attribute__((address_space(1)));
+template <typename V> using GOut = V __attribute__((address_space(1)));
+
+template <typename V> __device__ __forceinline__ V ld_elem(const V *p) { return *p; }
+
+template <typename V> __device__ __forceinline__ V ld_elem(GIn<V> *p)
+{
+    typedef decltype(V::x) R2 __attribute__((ext_vector_type(2)));
+    const R2 r = *reinterpret_cast<GIn<R2> *>(p);
+    return c_make<V>(r.x, r.y);
+}
+
+template <typename V> __device__ __forceinline__ void st_elem(GOut<V> *p, V v)
+{
+    typedef decltype(V::x) R2 __attribute__((ext_vector_type(2)));
+    R2 r;
+    r.x = v.x;
+    r.y = v.y;
+    *reinterpret_cast<GOut<R2> *>(p) = r;
+}
 template <typename V> __device__ __forceinline__ void c_store(V *p, V a, bool nt)
 {
     if (nt) c_store_nt(p, a);
@@ -576,7 +601,10 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
     static_assert(T * E == N, "E must divide N");
     V tb[NP > 0 ? NP : 1][2 * E];
 
-    __device__ __forceinline__ void load(const V *__restrict__ tw, int t)
+    __device__ __forceinline__ void load(const V *__restrict__ tw, int t) { load_from(tw, t); }
+    __device__ __forceinline__ void load(GIn<V> *tw, int t) { load_from(tw, t); }
+
+    template <typename P> __device__ __forceinline__ void load_from(P tw, int t)
     {
         static_for<NP>([&](auto pc) {
             constexpr int p = decltype(pc)::value;
@@ -589,8 +617,8 @@ template <typename V, int N, int E, int... Rs> struct CtStockham {
 #pragma unroll
                 for (int jb = 0; jb < NB; ++jb) {
                     const int k = (t + jb * T) % Ns;
-                    tb[p][2 * jb] = tw[k * STEP];
-                    tb[p][2 * jb + 1] = (R > 5) ? tw[4 * k * STEP] : tb[p][2 * jb];
+                    tb[p][2 * jb] = ld_elem(tw + k * STEP);
+                    tb[p][2 * jb + 1] = (R > 5) ? ld_elem(tw + 4 * k * STEP) : tb[p][2 * jb];
                 }
             }
         });

@@ -2,10 +2,11 @@
 // in one persistent launch (kernel afft_ragged<Real, EMAX>).
 //
 // The host sorts the transforms into length classes and cuts the classes that fit one workgroup
-// into work items, ordered by decreasing cost.  The resident workgroups walk the device-side
-// item list round-robin; no workgroup ever waits for another, so the launch always drains, and
-// there is no device-side state to reset between launches: the launch replays as a captured
-// hipGraph.  Longer classes run as plans of the batched engine over the rows of the class
+// into work items, ordered by decreasing cost.  The workgroups walk the device-side item list
+// round-robin (a grid of up to four times the resident workgroups: one item each for all but the
+// largest batches, so a workgroup that has issued its stores retires and the next one's loads
+// overlap them); no workgroup ever waits for another, so the launch always drains, and there is
+// no device-side state to reset between launches: the launch replays as a captured hipGraph.  Longer classes run as plans of the batched engine over the rows of the class
 // (gathered by offset), in the same stream and graph.
 //
 // Item kinds:
@@ -118,20 +119,23 @@ __device__ __noinline__ void run_ct(const RaggedArgs &a, const RaggedClass &K, c
     const bool blue = K.kind == kBlue;
     const int n = K.L; // the transform's own length: N, or the chirp-z length below N
     const int64_t off = active ? a.moffs[it.first + c] : 0;
-    const V *in = reinterpret_cast<const V *>(a.in);
-    V *out = reinterpret_cast<V *>(a.out);
-    const V *chirp = reinterpret_cast<const V *>(K.chirp);
+    // global-memory pointers: the item's stores then count only against the vector-memory
+    // counter, so neither the LDS waits of the next item nor the barrier ending this one wait for
+    // them (through the generic pointers of a non-inlined function they would be flat stores)
+    GIn<V> *in = (GIn<V> *)a.in;
+    GOut<V> *out = (GOut<V> *)a.out;
+    GIn<V> *chirp = (GIn<V> *)K.chirp;
     CT ct;
-    ct.load(reinterpret_cast<const V *>(K.tw), t);
+    ct.load((GIn<V> *)K.tw, t);
     V reg[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         const int m = t + i * T;
         V v = c_make<V>(0, 0);
         if (active && m < n) {
-            v = in[off + m];
+            v = ld_elem(in + off + m);
             if (a.conj) v = c_conj(v);
-            if (blue) v = c_mul(v, chirp[m]);
+            if (blue) v = c_mul(v, ld_elem(chirp + m));
         }
         reg[i] = v;
     }
@@ -140,9 +144,9 @@ __device__ __noinline__ void run_ct(const RaggedArgs &a, const RaggedClass &K, c
     __asm__ volatile("" : "+v"(tl));
     ct.template run<true, kWave, true>(reg, slot, 1, tl);
     if (blue) {
-        const V *filt = reinterpret_cast<const V *>(K.filt);
+        GIn<V> *filt = (GIn<V> *)K.filt;
 #pragma unroll
-        for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], filt[t + i * T]));
+        for (int i = 0; i < E; ++i) reg[i] = c_conj(c_mul(reg[i], ld_elem(filt + t + i * T)));
         // the second transform's first exchange writes the slot the first one read last: in
         // order within the wavefront, or behind the pass's leading barrier
         ct.template run<true, kWave, true>(reg, slot, 1, tl);
@@ -153,9 +157,9 @@ __device__ __noinline__ void run_ct(const RaggedArgs &a, const RaggedClass &K, c
         for (int i = 0; i < E; ++i) {
             const int m = t + i * T;
             if (m < n) {
-                V v = blue ? c_mul(c_conj(reg[i]), chirp[m]) : reg[i];
+                V v = blue ? c_mul(c_conj(reg[i]), ld_elem(chirp + m)) : reg[i];
                 if (a.conj) v = c_conj(v);
-                out[off + m] = c_scale(v, sc);
+                st_elem(out + off + m, c_scale(v, sc));
             }
         }
     }
@@ -191,7 +195,7 @@ __global__ void __launch_bounds__(kRaggedBlock) __attribute__((amdgpu_waves_per_
 #undef AFFT_RCT
                     default: break;
                     }
-                    __syncthreads(); // LDS (transforms and descriptor) reuse by the next item
+                    lds_sync(); // LDS (transforms and descriptor) reuse by the next item
                     continue;
                 }
             }

@@ -335,7 +335,11 @@ static int build_ragged(RaggedPlan &rp, const int64_t *lengths, const DeviceInfo
         rp.ws_bytes += workspace_bytes(lp->plan);
     }
     const int64_t resident = (int64_t)dev.cus * ragged_occupancy(rp.dtype, rp.lds);
-    rp.grid = std::max<int64_t>(1, std::min<int64_t>(resident, rp.nitems));
+    // up to four times the resident workgroups: one item each unless the batch has more items,
+    // so that a workgroup retires once its stores are issued and a new one's loads overlap them
+    // (one resident round walking the list: 4096 x 4096 137 us, the 10,000-transform batch's
+    // persistent classes 67 us and the 2^27 batch 965 us untuned, against 143 / 65 / 921 us)
+    rp.grid = std::max<int64_t>(1, std::min<int64_t>(4 * resident, rp.nitems));
     if (hipStreamSynchronize(s) != hipSuccess) return AFFT_ERROR_HIP;
     return AFFT_SUCCESS;
 }
