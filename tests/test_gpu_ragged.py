@@ -132,3 +132,20 @@ def test_ragged_compiled_and_table_layouts(gpu, monkeypatch, mode):
         outs.append(y)
         plan.close()
     assert O.rel_l2(outs[0].cpu().numpy(), outs[1].cpu().numpy()) <= tolerance(False, 4096)
+
+
+def test_ragged_items_beyond_the_grid(gpu):
+    # more work items than the persistent grid holds (up to four rounds of resident workgroups,
+    # 3072 on 256 CUs): 3500 transforms of 4096 points (one item each) and a mixture of shorter
+    # classes, so workgroups walk several items of different classes in turn
+    rng = np.random.default_rng(21)
+    short = [n for n in config_lengths(600, seed=21) if n <= 4096]
+    lengths = [4096] * 3500 + [2048] * 300 + short
+    lengths = [int(n) for n in rng.permutation(lengths)]
+    total = sum(lengths)
+    x = torch.from_numpy(O.uniform(22, (total,))).to(torch.complex64).to(gpu)
+    plan = afft.RaggedPlan(lengths, torch.complex64)
+    for mode in (Mode.Forward, Mode.Inverse):
+        y = plan.execute(mode, x)
+        check_rows(y, x, lengths, mode, torch.complex64)
+    plan.close()
