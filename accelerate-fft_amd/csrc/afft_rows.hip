@@ -457,6 +457,28 @@ int row_variants(int dtype, int n)
     return v;
 }
 
+// The default variant of a length: the three-waves-per-SIMD layout (variant 1) for Complex Float
+// 1024, 2048 and 4096, whose tiles pair best with one tile per workgroup on large batches
+// (16384 x 4096: 176.7 us against 178.9 us for variant 0; 128 MiB batches with four resident
+// rounds 45.5-46.8 against 46.3-48.3 us) and which is also the fastest persistent layout of
+// 4096 at 64 MiB (21.7 against 25.2 us); variant 0 elsewhere.
+int row_default_variant(int dtype, int n)
+{
+    if (dtype == 0 && (n == 1024 || n == 2048 || n == 4096) && row_variants(dtype, n) > 1) return 1;
+    return 0;
+}
+
+// Whether a batch of these rows runs better with more workgroups than are resident: each one
+// retires once its stores are issued and the dispatcher starts the next, whose loads overlap them.
+// Measured on the compiled lengths up to 4096 Complex Float and 2048 Complex Double (the longer
+// rows, one or two to a workgroup, lose: 8192 x 8192 236 -> 275 us, CD 4096 x 8192 189 -> 250 us).
+bool row_grid_beyond_resident(int dtype, int n, int variant)
+{
+    const RowEntry *e = find_row(dtype, n, variant);
+    if (!e || e->split || e->tr) return false;
+    return dtype == 0 ? n <= 4096 : n <= 2048;
+}
+
 bool row_split(int dtype, int n)
 {
     const RowEntry *e = find_row(dtype, n, 0);

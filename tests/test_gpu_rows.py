@@ -118,3 +118,26 @@ def test_transposed_rows_at_every_output_alignment(gpu, n):
         assert torch.equal(out, ref), off
         assert torch.isnan(buf[:off]).all() and torch.isnan(buf[off + total:]).all(), off
     plan.close()
+
+
+@pytest.mark.parametrize("dtype,n,rows", [
+    (torch.complex64, 256, 262144 + 7),    # 512 MiB: one tile per workgroup
+    (torch.complex64, 1024, 40000 + 3),    # 312 MiB: one tile per workgroup, the default variant 1
+    (torch.complex64, 2048, 9000 + 1),     # 140 MiB: four resident rounds
+    (torch.complex128, 512, 40000 + 5),    # 312 MiB
+    (torch.complex128, 2048, 5000 + 3),    # 156 MiB: four resident rounds
+])
+def test_rows_grid_beyond_resident(gpu, dtype, n, rows):
+    # large batches launch more row workgroups than are resident (four rounds beyond 128 MiB, one
+    # tile per workgroup beyond 256 MiB), ending in a partial tile; sampled rows against the oracle
+    x = torch.from_numpy(O.uniform(41, (rows, n))).to(dtype).to(gpu)
+    pick = np.r_[0:24, rows // 2 - 12:rows // 2 + 12, rows - 24:rows]
+    xr = x[pick].cpu().numpy().astype(np.complex128)
+    tol = tolerance(dtype == torch.complex128, n)
+    p = afft.Plan((rows, n), dtype, "innermost")
+    assert p.info()["launches"] == 1
+    y = p.execute(x, -1, 1.0)
+    assert O.rel_l2(y[pick].cpu().numpy(), O.fft(O.Mode.Forward, xr)) <= tol
+    z = p.execute(x, +1, 1.0 / n)
+    assert O.rel_l2(z[pick].cpu().numpy(), O.fft(O.Mode.Inverse, xr)) <= tol
+    p.close()
