@@ -517,9 +517,12 @@ static int conv_vjp(int dtype, int64_t batch, int64_t h, int64_t w, const void *
     if ((st = cached_plan(rp, dtype, 2, rows, AFFT_AXES_INNERMOST, dev))) return st;
     DeviceInfo di;
     if ((st = device_info(dev, di))) return st;
-    // groups of images per column tile: enough workgroups to cover the chip twice
+    // groups of images per column tile: enough workgroups to cover the chip once (one workgroup of
+    // this kernel fits a CU).  Each group's gH partial is one more h x w array written and read
+    // back: on 64 x 512^2, 8 groups took the squared-error VJP 303-308 us, 16 groups 303-315 us,
+    // 32 groups 315-319 us and 64 groups 346-354 us
     const int64_t tiles = (w + cs.C - 1) / cs.C;
-    int groups = (int)std::min<int64_t>(batch, std::max<int64_t>(1, (2 * di.cus + tiles - 1) / tiles));
+    int groups = (int)std::min<int64_t>(batch, std::max<int64_t>(1, (di.cus + tiles - 1) / tiles));
     const size_t arr = (size_t)(batch * h * w) * esz;
     const size_t part = (size_t)groups * (size_t)(h * w) * esz;
     void *ws = nullptr;
