@@ -188,8 +188,9 @@ struct RowArgs {
 constexpr int kRowTransposed = -2; // the variant index of a length's transposed-store form
 constexpr int kRowResidual = -3;   // the variant index of a length's residual-load form
 int row_variants(int dtype, int n);
-// the variant a plan takes when neither AFFT_ROW_VARIANT nor the autotuner chooses one
-int row_default_variant(int dtype, int n);
+// the variant a plan takes for a batch of `bytes` when neither AFFT_ROW_VARIANT nor the autotuner
+// chooses one
+int row_default_variant(int dtype, int n, int64_t bytes);
 // true when a large batch of these rows runs faster with more workgroups than are resident
 bool row_grid_beyond_resident(int dtype, int n, int variant);
 // true when the default layout of (dtype, n) is a split-exchange one (one row per workgroup)

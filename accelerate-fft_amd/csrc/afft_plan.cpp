@@ -595,7 +595,7 @@ struct Builder {
 
     // an explicit AFFT_ROW_VARIANT, else the variant forced by the autotuner or tuned, else the
     // default layout
-    int pick_row_variant(int64_t n)
+    int pick_row_variant(int64_t n, int64_t nrows)
     {
         const int nv = row_variants(plan.dtype, (int)n);
         int variant = -1;
@@ -603,7 +603,7 @@ struct Builder {
             variant = std::atoi(v);
         } else {
             // the default first: the autotuner keeps it unless another is measurably faster
-            const int def = row_default_variant(plan.dtype, (int)n);
+            const int def = row_default_variant(plan.dtype, (int)n, nrows * n * (int64_t)esz());
             std::vector<int> cands = {def};
             for (int v = 0; v < nv; ++v)
                 if (v != def) cands.push_back(v);
@@ -732,7 +732,7 @@ struct Builder {
             return;
         }
         if (row_variants(plan.dtype, (int)n2) > 0 && !getenv_flag("AFFT_NO_ROWS")) {
-            add_rows(n2, o * n1, cur, cur, pick_row_variant(n2));
+            add_rows(n2, o * n1, cur, cur, pick_row_variant(n2, o * n1));
         } else {
             Addr ra{};
             ra.cnt0 = 1;
@@ -854,7 +854,7 @@ struct Builder {
         a.es = s_;
         const int64_t limit = dev.max_lds_len[plan.dtype];
         switch (axis_kind(n, s_, o)) {
-        case AxisKind::kRows: add_rows(n, ntrans, src, cur, pick_row_variant(n)); return cur;
+        case AxisKind::kRows: add_rows(n, ntrans, src, cur, pick_row_variant(n, ntrans)); return cur;
         case AxisKind::kSingle: add_stockham(n, ntrans, a, a, strided, src, cur); return cur;
         case AxisKind::kBluestein: add_bluestein(n, ntrans, a, a, strided, src, cur); return cur;
         case AxisKind::kSplit: break;

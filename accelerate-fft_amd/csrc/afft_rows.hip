@@ -457,14 +457,17 @@ int row_variants(int dtype, int n)
     return v;
 }
 
-// The default variant of a length: the three-waves-per-SIMD layout (variant 1) for Complex Float
-// 1024, 2048 and 4096, whose tiles pair best with one tile per workgroup on large batches
-// (16384 x 4096: 176.7 us against 178.9 us for variant 0; 128 MiB batches with four resident
-// rounds 45.5-46.8 against 46.3-48.3 us) and which is also the fastest persistent layout of
-// 4096 at 64 MiB (21.7 against 25.2 us); variant 0 elsewhere.
-int row_default_variant(int dtype, int n)
+// The default variant of a length for a batch of `bytes`: the three-waves-per-SIMD layout
+// (variant 1) for Complex Float 4096, and for 1024 and 2048 from 128 MiB, whose tiles pair best
+// with one tile per workgroup on large batches (16384 x 4096: 176.7 us against 178.9 us for
+// variant 0; 128 MiB batches with four resident rounds 45.5-46.8 against 46.3-48.3 us).  It is also
+// the fastest persistent layout of 4096 at 64 MiB (21.7 against 25.2 us), where 1024 and 2048 run
+// variant 0 (24.2 / 25.5 against 25.0 / 25.8 us); variant 0 elsewhere.
+int row_default_variant(int dtype, int n, int64_t bytes)
 {
-    if (dtype == 0 && (n == 1024 || n == 2048 || n == 4096) && row_variants(dtype, n) > 1) return 1;
+    if (dtype != 0 || row_variants(dtype, n) < 2) return 0;
+    if (n == 4096) return 1;
+    if ((n == 1024 || n == 2048) && bytes >= ((int64_t)128 << 20)) return 1;
     return 0;
 }
 
